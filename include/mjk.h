@@ -1,0 +1,130 @@
+/*
+ * mjk.h -- C ABI of the MI355X direct Coulomb/exchange (J/K) engine.
+ *
+ * Drop-in boundary for PySCF's direct J/K hot path.  The reference crosses
+ * from Python into C once per J/K build through ctypes:
+ *
+ *   void CVHFnr_direct_drv(int (*intor)(), void (*fdot)(), JKOperator **jkop,
+ *                          double **dms, double **vjk, int n_dm, int ncomp,
+ *                          int *shls_slice, int *ao_loc, CINTOpt *cintopt,
+ *                          CVHFOpt *vhfopt, int *atm, int natm, int *bas,
+ *                          int nbas, double *env);
+ *   (pyscf/lib/vhf/nr_direct.h:74-78, called from pyscf/scf/_vhf.py:540-639)
+ *
+ * with the screening context CVHFOpt built by _VHFOpt
+ * (pyscf/lib/vhf/optimizer.h:23-34, pyscf/scf/_vhf.py:151-264).  Function
+ * pointers to libcint and to the digestion operators do not cross a device
+ * boundary, so the same work is exposed as a handle-based API: the basis
+ * (the reference's atm/bas/env arrays, unchanged) is uploaded once and
+ * keeps shell pairs, Schwarz bounds and work buffers resident in HBM; each
+ * J/K build passes densities and receives J and K.
+ *
+ * Conventions (all arrays C-contiguous):
+ *   atm  int32 [natm][6], bas int32 [nbas][8], env float64 -- PySCF layout
+ *        (pyscf/gto/mole.py:58-88), spherical AOs, ao_loc = cumsum((2l+1)*nctr).
+ *   dms  float64 [n_dm][nao][nao]; vj, vk float64 [n_dm][nao][nao].
+ *   For a batch of molecules, dms/vj/vk hold the molecules' [nao_m][nao_m]
+ *   blocks back to back inside every density slot.
+ *   hermi: 0 any dm, 1 symmetric dm (K mirrored from the lower triangle),
+ *          2 anti-symmetric (upper triangle of K = -lower).  J is always the
+ *          symmetric Coulomb matrix (pyscf/scf/_vhf.py:405-464).
+ *   direct_scf_tol <= 0: no screening (module-level get_jk without an
+ *          optimizer, pyscf/scf/hf.py:948-1022); > 0: the CVHFnrs8_prescreen
+ *          predicate with Schwarz q_cond and dm_cond
+ *          (pyscf/lib/vhf/optimizer.c:90-117, 408-454, 494-518).
+ * Return codes: 0 success, < 0 error (mjk_last_error gives the message);
+ * the library never exits the process.
+ */
+#ifndef MJK_H
+#define MJK_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MJK_OK 0
+#define MJK_EINVAL (-1)       /* bad argument or shape */
+#define MJK_ENOMEM (-2)       /* request exceeds the context's HBM budget */
+#define MJK_EDEVICE (-3)      /* HIP runtime or kernel failure */
+#define MJK_EUNSUPPORTED (-4) /* angular momentum or feature not built */
+
+typedef struct mjk_ctx mjk_ctx;
+typedef struct mjk_basis mjk_basis;
+
+typedef struct {
+    double ms_total;            /* host wall time of the call */
+    double ms_quartets;         /* GPU time of the ERI/J/K kernels (HIP events) */
+    double ms_transform;        /* GPU time of dm_cond and sph<->cart transforms */
+    long long quartets;         /* evaluated (post-screening) shell quartets */
+    long long prim_quartets;    /* evaluated primitive quartets */
+    double flops;               /* algorithmic FLOPs of the evaluated quartets */
+} mjk_stats;
+
+/* Context on one HIP device; hbm_budget = 0 means 80% of free memory.
+ * Replaces the process-wide state the reference keeps in OpenMP/ctypes
+ * (pyscf/lib/misc.py:95-107 load_library, pyscf/lib/misc.py:125-154). */
+int mjk_ctx_create(mjk_ctx** ctx, int device, size_t hbm_budget);
+void mjk_ctx_destroy(mjk_ctx* ctx);
+const char* mjk_last_error(const mjk_ctx* ctx);
+/* Bytes of HBM currently held by the context. */
+size_t mjk_ctx_hbm_used(const mjk_ctx* ctx);
+
+/* Upload one molecule's basis and build its shell pairs, Schwarz bounds
+ * (the q_cond of CVHFnr_int2e_q_cond, pyscf/lib/vhf/optimizer.c:408-454)
+ * and quartet tile lists for screening at `cutoff` (<= 0: none).
+ * Replaces _VHFOpt.__init__/init_cvhf_direct (pyscf/scf/_vhf.py:151-206). */
+int mjk_basis_upload(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas,
+                     const double* env, double cutoff, mjk_basis** out);
+
+/* Batch of nmol independent molecules in one handle (one launch per quartet
+ * class covers all of them).  No reference counterpart: the reference
+ * builds one molecule at a time. */
+int mjk_basis_upload_batch(mjk_ctx* ctx, int nmol, const int* natm, const int* nbas,
+                           const int* const* atm, const int* const* bas,
+                           const double* const* env, double cutoff, mjk_basis** out);
+void mjk_basis_destroy(mjk_basis* basis);
+
+/* nao of molecule `mol` (spherical AOs). */
+int mjk_basis_nao(const mjk_basis* basis, int mol);
+/* Total spherical elements of one density slot: sum_m nao_m^2. */
+long long mjk_basis_slot_size(const mjk_basis* basis);
+/* Schwarz bound q_cond [nbas][nbas] of molecule `mol` over the reference's
+ * shells (max over the contracted functions), for comparison with
+ * CVHFnr_int2e_q_cond (pyscf/lib/vhf/optimizer.c:408-454). */
+int mjk_basis_q_cond(mjk_ctx* ctx, const mjk_basis* basis, int mol, double* q_cond);
+
+/* J/K build from host arrays (synchronous).  Replaces the
+ * _vhf.direct -> CVHFnr_direct_drv(CVHFdot_nrs8, CVHFnrs8_ji_s2kl,
+ * CVHFnrs8_li_s2kj|s1kj) call chain (pyscf/scf/_vhf.py:405-464,
+ * pyscf/lib/vhf/nr_direct.c:355-476), including hermi_triu.
+ * fp32_tol > 0 enables mixed precision (quartets whose bound q_ij q_kl
+ * max|D| < fp32_tol are evaluated in FP32); 0 keeps every quartet in FP64. */
+int mjk_get_jk(mjk_ctx* ctx, mjk_basis* basis, const double* dms, int n_dm, int hermi,
+               int with_j, int with_k, double direct_scf_tol, double fp32_tol,
+               double* vj, double* vk, mjk_stats* stats);
+
+/* Same, with dms/vj/vk already in device memory (HBM-resident inputs).
+ * Asynchronous on the context's stream unless stats != NULL. */
+int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n_dm, int hermi,
+                      int with_j, int with_k, double direct_scf_tol, double fp32_tol,
+                      double* d_vj, double* d_vk, mjk_stats* stats);
+
+/* Device memory helpers (for callers without another GPU runtime). */
+int mjk_device_alloc(mjk_ctx* ctx, size_t bytes, void** ptr);
+int mjk_device_free(mjk_ctx* ctx, void* ptr);
+int mjk_memcpy_h2d(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
+int mjk_memcpy_d2h(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
+int mjk_synchronize(mjk_ctx* ctx);
+
+/* Rys roots x_i (= t_i^2) and weights of the interpolation table, evaluated
+ * on the host (table validation).  n <= 9.  No reference counterpart (the
+ * reference's roots live inside libcint). */
+int mjk_rys_roots(int n, double T, double* x, double* w);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MJK_H */

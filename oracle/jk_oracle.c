@@ -1,0 +1,887 @@
+/*
+ * ORACLE — TEST INFRASTRUCTURE ONLY.  CPU restatement of PySCF's direct J/K
+ * path.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load this library, and only as the checker / CPU baseline; the
+ * product (pyscf_amd) never calls it.
+ *
+ * What it restates (reference paths relative to the PySCF tree):
+ *   - s8 shell-quartet enumeration i>=j, k<=i, l<=k, (k==i => l<=j)
+ *                                              lib/vhf/nr_direct.c:182-230
+ *   - quartet prescreen predicate              lib/vhf/optimizer.c:90-117
+ *   - Schwarz q_cond (sqrt max |(ij|ij)|, floor 1e-100)
+ *                                              lib/vhf/optimizer.c:408-454
+ *   - dm_cond = 0.5 max(|D_ij|+|D_ji|)         lib/vhf/optimizer.c:494-518
+ *   - J 'ji->s2kl' and K 'li->s1kj' digestion (all 8 permutations of each
+ *     unique quartet)                          lib/vhf/nr_direct_dot.c:1293-1647
+ *   - int2e_sph block layout buf[i + di*(j + dj*(k + dk*l))]
+ *                                              lib/vhf/nr_direct_dot.c:209-232
+ * The ERI itself comes from libcint v6.1.3 (github.com/sunqm/libcint, pinned
+ * at pyscf/lib/CMakeLists.txt:175-190), absent here.  Its published algorithm
+ * (Q. Sun, J. Comput. Chem. 36, 1664 (2015)) is Rys quadrature
+ * (Rys, Dupuis, King, J. Comput. Chem. 4, 154 (1983)); restated here with
+ * roots/weights from a discretised Stieltjes procedure plus the Golub-Welsch
+ * eigenproblem (Gautschi, "Orthogonal Polynomials", 2004, sec. 2.2;
+ * Golub & Welsch, Math. Comp. 23, 221 (1969)) and the Gauss-Hermite
+ * large-T limit.  AO conventions: pyscf/gto/mole.py:125-187 and the real
+ * solid harmonics of pyscf/symm/test/test_sph.py:43-100 (p = x,y,z).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_MAXL 4
+#define ORC_MAXROOTS 9
+#define ORC_NGL 96
+#define ATM_SLOTS 6
+#define BAS_SLOTS 8
+#define PTR_COORD 1
+#define ANG_OF 1
+#define NPRIM_OF 2
+#define NCTR_OF 3
+#define PTR_EXP 5
+#define PTR_COEFF 6
+#define ATOM_OF 0
+
+static double gl_t[ORC_NGL], gl_w[ORC_NGL];
+static double herm_x[ORC_MAXROOTS + 1][ORC_MAXROOTS];
+static double herm_w[ORC_MAXROOTS + 1][ORC_MAXROOTS];
+static int orc_ready = 0;
+
+/* Symmetric tridiagonal eigenproblem by implicit-shift QL.  d: diagonal
+ * (eigenvalues on return), e[i]: coupling of i and i+1 (destroyed),
+ * z: first row of the eigenvector matrix (starts as e_0). */
+static void tridiag_ql(int n, double *d, double *e, double *z)
+{
+        int l, m, i, it;
+        for (i = 0; i < n; i++) z[i] = (i == 0) ? 1. : 0.;
+        e[n - 1] = 0.;
+        for (l = 0; l < n; l++) {
+                for (it = 0; it < 60; it++) {
+                        for (m = l; m < n - 1; m++) {
+                                double dd = fabs(d[m]) + fabs(d[m + 1]);
+                                if (fabs(e[m]) <= 1e-17 * dd) break;
+                        }
+                        if (m == l) break;
+                        double g = (d[l + 1] - d[l]) / (2. * e[l]);
+                        double r = hypot(g, 1.);
+                        g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+                        double s = 1., c = 1., p = 0.;
+                        int underflow = 0;
+                        for (i = m - 1; i >= l; i--) {
+                                double f = s * e[i];
+                                double b = c * e[i];
+                                r = hypot(f, g);
+                                e[i + 1] = r;
+                                if (r == 0.) {
+                                        d[i + 1] -= p;
+                                        e[m] = 0.;
+                                        underflow = 1;
+                                        break;
+                                }
+                                s = f / r;
+                                c = g / r;
+                                g = d[i + 1] - p;
+                                r = (d[i] - g) * s + 2. * c * b;
+                                p = s * r;
+                                d[i + 1] = g + p;
+                                g = c * r - b;
+                                f = z[i + 1];
+                                z[i + 1] = s * z[i] + c * f;
+                                z[i] = c * z[i] - s * f;
+                        }
+                        if (underflow) continue;
+                        d[l] -= p;
+                        e[l] = g;
+                        e[m] = 0.;
+                }
+        }
+}
+
+static void sort_pairs(int n, double *x, double *w)
+{
+        int i, j;
+        for (i = 1; i < n; i++) {
+                double xv = x[i], wv = w[i];
+                for (j = i - 1; j >= 0 && x[j] > xv; j--) {
+                        x[j + 1] = x[j];
+                        w[j + 1] = w[j];
+                }
+                x[j + 1] = xv;
+                w[j + 1] = wv;
+        }
+}
+
+/* Gauss-Legendre rule on [0,1] by Newton iteration on P_N. */
+static void init_gauss_legendre(void)
+{
+        int N = ORC_NGL, i, k;
+        for (i = 0; i < N; i++) {
+                double x = cos(M_PI * (i + 0.75) / (N + 0.5));
+                double dp = 0;
+                for (k = 0; k < 100; k++) {
+                        double p0 = 1., p1 = x;
+                        int j;
+                        for (j = 2; j <= N; j++) {
+                                double p2 = ((2. * j - 1.) * x * p1 - (j - 1.) * p0) / j;
+                                p0 = p1;
+                                p1 = p2;
+                        }
+                        dp = N * (x * p1 - p0) / (x * x - 1.);
+                        double dx = p1 / dp;
+                        x -= dx;
+                        if (fabs(dx) < 1e-16) break;
+                }
+                gl_t[i] = 0.5 * (1. - x);
+                gl_w[i] = 1. / ((1. - x * x) * dp * dp);
+        }
+}
+
+/* Half-range Gauss-Hermite rules: the positive roots s_i of H_2n give
+ * int_0^inf f(s^2) exp(-s^2) ds = sum_i W_i f(s_i^2). */
+static void init_hermite(void)
+{
+        int n;
+        for (n = 1; n <= ORC_MAXROOTS; n++) {
+                int N = 2 * n, i;
+                double d[2 * ORC_MAXROOTS], e[2 * ORC_MAXROOTS], z[2 * ORC_MAXROOTS];
+                for (i = 0; i < N; i++) {
+                        d[i] = 0.;
+                        e[i] = (i < N - 1) ? sqrt(0.5 * (i + 1)) : 0.;
+                }
+                tridiag_ql(N, d, e, z);
+                double xs[2 * ORC_MAXROOTS], ws[2 * ORC_MAXROOTS];
+                for (i = 0; i < N; i++) {
+                        xs[i] = d[i];
+                        ws[i] = sqrt(M_PI) * z[i] * z[i];
+                }
+                sort_pairs(N, xs, ws);
+                for (i = 0; i < n; i++) {
+                        herm_x[n][i] = xs[n + i] * xs[n + i];
+                        herm_w[n][i] = ws[n + i];
+                }
+        }
+}
+
+void orc_init(void)
+{
+        if (orc_ready) return;
+        init_gauss_legendre();
+        init_hermite();
+        orc_ready = 1;
+}
+
+/* Rys roots x_i = t_i^2 in (0,1) and weights w_i for the weight
+ * exp(-T t^2) on t in [0,1]:  sum_i w_i x_i^m = F_m(T) for m < 2n. */
+void orc_rys_roots(int n, double T, double *x, double *w)
+{
+        int i, j;
+        if (T >= 36. + 6. * n) {
+                double s = 1. / sqrt(T);
+                for (i = 0; i < n; i++) {
+                        x[i] = herm_x[n][i] / T;
+                        w[i] = herm_w[n][i] * s;
+                }
+                return;
+        }
+        double xk[ORC_NGL], mk[ORC_NGL], pc[ORC_NGL], pp[ORC_NGL];
+        double alpha[ORC_MAXROOTS], beta[ORC_MAXROOTS];
+        for (i = 0; i < ORC_NGL; i++) {
+                double t = gl_t[i];
+                xk[i] = t * t;
+                mk[i] = gl_w[i] * exp(-T * xk[i]);
+                pc[i] = 1.;
+                pp[i] = 0.;
+        }
+        double nrm_prev = 1.;
+        for (j = 0; j < n; j++) {
+                double nrm = 0., xn = 0.;
+                for (i = 0; i < ORC_NGL; i++) {
+                        double v = mk[i] * pc[i] * pc[i];
+                        nrm += v;
+                        xn += v * xk[i];
+                }
+                alpha[j] = xn / nrm;
+                beta[j] = (j == 0) ? nrm : nrm / nrm_prev;
+                nrm_prev = nrm;
+                for (i = 0; i < ORC_NGL; i++) {
+                        double pn = (xk[i] - alpha[j]) * pc[i] - ((j == 0) ? 0. : beta[j] * pp[i]);
+                        pp[i] = pc[i];
+                        pc[i] = pn;
+                }
+        }
+        double d[ORC_MAXROOTS], e[ORC_MAXROOTS], z[ORC_MAXROOTS];
+        for (j = 0; j < n; j++) {
+                d[j] = alpha[j];
+                e[j] = (j < n - 1) ? sqrt(beta[j + 1]) : 0.;
+        }
+        tridiag_ql(n, d, e, z);
+        for (j = 0; j < n; j++) {
+                x[j] = d[j];
+                w[j] = beta[0] * z[j] * z[j];
+        }
+        sort_pairs(n, x, w);
+}
+
+/* ---------------- Cartesian -> real solid harmonics ---------------- */
+/* Cartesian order per shell: for lx = l..0, for ly = l-lx..0 (libcint order:
+ * d = xx,xy,xz,yy,yz,zz).  Spherical order m = -l..l, except p = x,y,z. */
+static int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+static double c2s_tab[ORC_MAXL + 1][2 * ORC_MAXL + 1][(ORC_MAXL + 1) * (ORC_MAXL + 2) / 2];
+
+static int cart_index(int l, int lx, int ly)
+{
+        /* position of (lx,ly,lz) in the order above */
+        int idx = 0, a;
+        for (a = l; a > lx; a--) idx += l - a + 1;
+        return idx + (l - lx - ly);
+}
+
+static void init_c2s(void)
+{
+        memset(c2s_tab, 0, sizeof(c2s_tab));
+        double pi = M_PI;
+        c2s_tab[0][0][0] = 0.5 / sqrt(pi);
+        double cp = 0.5 * sqrt(3. / pi);
+        c2s_tab[1][0][0] = cp;
+        c2s_tab[1][1][1] = cp;
+        c2s_tab[1][2][2] = cp;
+        /* l = 2 */
+        double a = 0.5 * sqrt(15. / pi), b = 0.25 * sqrt(5. / pi), c = 0.25 * sqrt(15. / pi);
+        c2s_tab[2][0][cart_index(2, 1, 1)] = a;              /* xy */
+        c2s_tab[2][1][cart_index(2, 0, 1)] = a;              /* yz */
+        c2s_tab[2][2][cart_index(2, 0, 0)] = 2 * b;          /* 3z^2-r^2 */
+        c2s_tab[2][2][cart_index(2, 2, 0)] = -b;
+        c2s_tab[2][2][cart_index(2, 0, 2)] = -b;
+        c2s_tab[2][3][cart_index(2, 1, 0)] = a;              /* xz */
+        c2s_tab[2][4][cart_index(2, 2, 0)] = c;              /* x^2-y^2 */
+        c2s_tab[2][4][cart_index(2, 0, 2)] = -c;
+        /* l = 3 */
+        double f3 = 0.125 * sqrt(70. / pi), f2 = 0.5 * sqrt(105. / pi);
+        double f1 = 0.125 * sqrt(42. / pi), f0 = 0.25 * sqrt(7. / pi);
+        double g2 = 0.25 * sqrt(105. / pi);
+        c2s_tab[3][0][cart_index(3, 2, 1)] = 3 * f3;         /* y(3x^2-y^2) */
+        c2s_tab[3][0][cart_index(3, 0, 3)] = -f3;
+        c2s_tab[3][1][cart_index(3, 1, 1)] = f2;             /* xyz */
+        c2s_tab[3][2][cart_index(3, 0, 1)] = 4 * f1;         /* y(4z^2-x^2-y^2) */
+        c2s_tab[3][2][cart_index(3, 2, 1)] = -f1;
+        c2s_tab[3][2][cart_index(3, 0, 3)] = -f1;
+        c2s_tab[3][3][cart_index(3, 0, 0)] = 2 * f0;         /* z(2z^2-3x^2-3y^2) */
+        c2s_tab[3][3][cart_index(3, 2, 0)] = -3 * f0;
+        c2s_tab[3][3][cart_index(3, 0, 2)] = -3 * f0;
+        c2s_tab[3][4][cart_index(3, 1, 0)] = 4 * f1;         /* x(4z^2-x^2-y^2) */
+        c2s_tab[3][4][cart_index(3, 3, 0)] = -f1;
+        c2s_tab[3][4][cart_index(3, 1, 2)] = -f1;
+        c2s_tab[3][5][cart_index(3, 2, 0)] = g2;             /* z(x^2-y^2) */
+        c2s_tab[3][5][cart_index(3, 0, 2)] = -g2;
+        c2s_tab[3][6][cart_index(3, 3, 0)] = f3;             /* x(x^2-3y^2) */
+        c2s_tab[3][6][cart_index(3, 1, 2)] = -3 * f3;
+}
+
+void orc_c2s(int l, double *out)
+{
+        /* out[m * ncart + c] */
+        int m, c;
+        for (m = 0; m < 2 * l + 1; m++)
+                for (c = 0; c < ncart(l); c++)
+                        out[m * ncart(l) + c] = c2s_tab[l][m][c];
+}
+
+/* ---------------- shell quartet ERI (Rys quadrature) ---------------- */
+typedef struct {
+        int l, nprim, nctr;
+        const double *exp, *coef;  /* coef[ctr*nprim + prim] */
+        double r[3];
+} Shell;
+
+static void get_shell(Shell *s, int ish, const int *atm, const int *bas, const double *env)
+{
+        const int *b = bas + ish * BAS_SLOTS;
+        s->l = b[ANG_OF];
+        s->nprim = b[NPRIM_OF];
+        s->nctr = b[NCTR_OF];
+        s->exp = env + b[PTR_EXP];
+        s->coef = env + b[PTR_COEFF];
+        const double *c = env + atm[b[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+        s->r[0] = c[0];
+        s->r[1] = c[1];
+        s->r[2] = c[2];
+}
+
+#define G1 (2 * ORC_MAXL + 1)
+#define H1 (ORC_MAXL + 1)
+
+/* Cartesian contracted block for one shell quartet.
+ * cart[((((ci*Nj + cj)*Nk + ck)*Nl + cl)] with Ni = nctr_i*ncart_i ordered
+ * (ctr, cart). */
+static void eri_cart(double *cart, const Shell *si, const Shell *sj, const Shell *sk, const Shell *sl)
+{
+        int li = si->l, lj = sj->l, lk = sk->l, ll = sl->l;
+        int ni = ncart(li), nj = ncart(lj), nk = ncart(lk), nl = ncart(ll);
+        int Ni = ni * si->nctr, Nj = nj * sj->nctr, Nk = nk * sk->nctr, Nl = nl * sl->nctr;
+        int nroots = (li + lj + lk + ll) / 2 + 1;
+        int lij = li + lj, lkl = lk + ll;
+        double AB[3], CD[3], rab2 = 0, rcd2 = 0;
+        int d, r, a, b, c, e;
+        for (d = 0; d < 3; d++) {
+                AB[d] = si->r[d] - sj->r[d];
+                CD[d] = sk->r[d] - sl->r[d];
+                rab2 += AB[d] * AB[d];
+                rcd2 += CD[d] * CD[d];
+        }
+        memset(cart, 0, sizeof(double) * Ni * Nj * Nk * Nl);
+        /* cartesian exponent tables */
+        int ix[4][15], iy[4][15], iz[4][15];
+        const int ls[4] = {li, lj, lk, ll};
+        for (a = 0; a < 4; a++) {
+                int n = 0, lx, ly;
+                for (lx = ls[a]; lx >= 0; lx--)
+                        for (ly = ls[a] - lx; ly >= 0; ly--) {
+                                ix[a][n] = lx;
+                                iy[a][n] = ly;
+                                iz[a][n] = ls[a] - lx - ly;
+                                n++;
+                        }
+        }
+        int nprim_q = ni * nj * nk * nl;
+        double *gp = malloc(sizeof(double) * nprim_q);
+        double g[3][G1][G1];
+        double I[ORC_MAXROOTS][3][H1][H1][H1][H1];
+        int pi, pj, pk, pl;
+        for (pi = 0; pi < si->nprim; pi++)
+        for (pj = 0; pj < sj->nprim; pj++) {
+                double ai = si->exp[pi], aj = sj->exp[pj];
+                double p = ai + aj;
+                double Kab = exp(-ai * aj / p * rab2);
+                double P[3];
+                for (d = 0; d < 3; d++) P[d] = (ai * si->r[d] + aj * sj->r[d]) / p;
+                for (pk = 0; pk < sk->nprim; pk++)
+                for (pl = 0; pl < sl->nprim; pl++) {
+                        double ak = sk->exp[pk], al = sl->exp[pl];
+                        double q = ak + al;
+                        double Kcd = exp(-ak * al / q * rcd2);
+                        double Q[3], PQ2 = 0;
+                        for (d = 0; d < 3; d++) {
+                                Q[d] = (ak * sk->r[d] + al * sl->r[d]) / q;
+                                PQ2 += (P[d] - Q[d]) * (P[d] - Q[d]);
+                        }
+                        double pq = p + q;
+                        double rho = p * q / pq;
+                        double T = rho * PQ2;
+                        double fac = 2. * pow(M_PI, 2.5) / (p * q * sqrt(pq)) * Kab * Kcd;
+                        double xr[ORC_MAXROOTS], wr[ORC_MAXROOTS];
+                        orc_rys_roots(nroots, T, xr, wr);
+                        for (r = 0; r < nroots; r++) {
+                                double t2 = xr[r];
+                                double B00 = 0.5 * t2 / pq;
+                                double B10 = 0.5 / p * (1. - t2 * q / pq);
+                                double B01 = 0.5 / q * (1. - t2 * p / pq);
+                                for (d = 0; d < 3; d++) {
+                                        double C00 = P[d] - si->r[d] + t2 * q / pq * (Q[d] - P[d]);
+                                        double Cp = Q[d] - sk->r[d] + t2 * p / pq * (P[d] - Q[d]);
+                                        double (*gd)[G1] = g[d];
+                                        gd[0][0] = (d == 2) ? wr[r] * fac : 1.;
+                                        if (lij > 0) gd[1][0] = C00 * gd[0][0];
+                                        for (a = 1; a < lij; a++)
+                                                gd[a + 1][0] = C00 * gd[a][0] + a * B10 * gd[a - 1][0];
+                                        for (c = 0; c < lkl; c++) {
+                                                for (a = 0; a <= lij; a++) {
+                                                        double v = Cp * gd[a][c];
+                                                        if (c > 0) v += c * B01 * gd[a][c - 1];
+                                                        if (a > 0) v += a * B00 * gd[a - 1][c];
+                                                        gd[a][c + 1] = v;
+                                                }
+                                        }
+                                        /* HRR bra then ket */
+                                        double hb[G1][H1][G1];
+                                        for (a = 0; a <= lij; a++)
+                                                for (c = 0; c <= lkl; c++) hb[a][0][c] = gd[a][c];
+                                        for (b = 1; b <= lj; b++)
+                                                for (a = 0; a <= lij - b; a++)
+                                                        for (c = 0; c <= lkl; c++)
+                                                                hb[a][b][c] = hb[a + 1][b - 1][c] + AB[d] * hb[a][b - 1][c];
+                                        for (a = 0; a <= li; a++)
+                                        for (b = 0; b <= lj; b++) {
+                                                double hk[G1][H1];
+                                                for (c = 0; c <= lkl; c++) hk[c][0] = hb[a][b][c];
+                                                for (e = 1; e <= ll; e++)
+                                                        for (c = 0; c <= lkl - e; c++)
+                                                                hk[c][e] = hk[c + 1][e - 1] + CD[d] * hk[c][e - 1];
+                                                for (c = 0; c <= lk; c++)
+                                                        for (e = 0; e <= ll; e++)
+                                                                I[r][d][a][b][c][e] = hk[c][e];
+                                        }
+                                }
+                        }
+                        int fi, fj, fk, fl, n = 0;
+                        for (fi = 0; fi < ni; fi++)
+                        for (fj = 0; fj < nj; fj++)
+                        for (fk = 0; fk < nk; fk++)
+                        for (fl = 0; fl < nl; fl++, n++) {
+                                double s = 0;
+                                for (r = 0; r < nroots; r++)
+                                        s += I[r][0][ix[0][fi]][ix[1][fj]][ix[2][fk]][ix[3][fl]]
+                                           * I[r][1][iy[0][fi]][iy[1][fj]][iy[2][fk]][iy[3][fl]]
+                                           * I[r][2][iz[0][fi]][iz[1][fj]][iz[2][fk]][iz[3][fl]];
+                                gp[n] = s;
+                        }
+                        int ci, cj, ck, cl;
+                        for (ci = 0; ci < si->nctr; ci++)
+                        for (cj = 0; cj < sj->nctr; cj++)
+                        for (ck = 0; ck < sk->nctr; ck++)
+                        for (cl = 0; cl < sl->nctr; cl++) {
+                                double cc = si->coef[ci * si->nprim + pi] * sj->coef[cj * sj->nprim + pj]
+                                          * sk->coef[ck * sk->nprim + pk] * sl->coef[cl * sl->nprim + pl];
+                                n = 0;
+                                for (fi = 0; fi < ni; fi++)
+                                for (fj = 0; fj < nj; fj++)
+                                for (fk = 0; fk < nk; fk++)
+                                for (fl = 0; fl < nl; fl++, n++) {
+                                        size_t off = ((((size_t)(ci * ni + fi) * Nj + cj * nj + fj) * Nk
+                                                      + ck * nk + fk) * Nl + cl * nl + fl);
+                                        cart[off] += cc * gp[n];
+                                }
+                        }
+                }
+        }
+        free(gp);
+}
+
+/* Spherical block in Fortran order out[i + di*(j + dj*(k + dk*l))],
+ * d = (2l+1)*nctr, AO order (ctr, m). */
+static void eri_sph(double *out, const Shell *sh[4])
+{
+        int Nc[4], Ns[4], a;
+        for (a = 0; a < 4; a++) {
+                Nc[a] = ncart(sh[a]->l) * sh[a]->nctr;
+                Ns[a] = (2 * sh[a]->l + 1) * sh[a]->nctr;
+        }
+        size_t ncart_tot = (size_t)Nc[0] * Nc[1] * Nc[2] * Nc[3];
+        double *cart = malloc(sizeof(double) * ncart_tot);
+        eri_cart(cart, sh[0], sh[1], sh[2], sh[3]);
+        /* transform index by index; layout kept C-order (i slowest) */
+        int dims[4] = {Nc[0], Nc[1], Nc[2], Nc[3]};
+        double *cur = cart;
+        for (a = 0; a < 4; a++) {
+                int l = sh[a]->l, nc = ncart(l), ns = 2 * l + 1, nctr = sh[a]->nctr;
+                size_t pre = 1, post = 1;
+                int b;
+                for (b = 0; b < a; b++) pre *= dims[b];
+                for (b = a + 1; b < 4; b++) post *= dims[b];
+                double *nxt = malloc(sizeof(double) * pre * Ns[a] * post);
+                size_t x, y;
+                int ctr, m, c;
+                for (x = 0; x < pre; x++)
+                for (ctr = 0; ctr < nctr; ctr++)
+                for (m = 0; m < ns; m++)
+                for (y = 0; y < post; y++) {
+                        double s = 0;
+                        for (c = 0; c < nc; c++)
+                                s += c2s_tab[l][m][c] * cur[(x * dims[a] + ctr * nc + c) * post + y];
+                        nxt[(x * Ns[a] + ctr * ns + m) * post + y] = s;
+                }
+                free(cur);
+                cur = nxt;
+                dims[a] = Ns[a];
+        }
+        /* C order (i slowest) -> Fortran order (i fastest) */
+        int i, j, k, l;
+        for (i = 0; i < Ns[0]; i++)
+        for (j = 0; j < Ns[1]; j++)
+        for (k = 0; k < Ns[2]; k++)
+        for (l = 0; l < Ns[3]; l++)
+                out[i + Ns[0] * (j + Ns[1] * (k + (size_t)Ns[2] * l))] =
+                        cur[(((size_t)i * Ns[1] + j) * Ns[2] + k) * Ns[3] + l];
+        free(cur);
+}
+
+static int c2s_ready = 0;
+static void ensure_init(void)
+{
+        if (!orc_ready) orc_init();
+        if (!c2s_ready) { init_c2s(); c2s_ready = 1; }
+}
+
+/* int2e_sph-like: one shell quartet, Fortran-order spherical block. */
+int orc_int2e_sph(double *out, const int *shls, const int *atm, int natm,
+                  const int *bas, int nbas, const double *env)
+{
+        ensure_init();
+        Shell s[4];
+        const Shell *sp[4];
+        int a;
+        for (a = 0; a < 4; a++) {
+                get_shell(&s[a], shls[a], atm, bas, env);
+                sp[a] = &s[a];
+        }
+        eri_sph(out, sp);
+        return 1;
+}
+
+/* Full (nao^4) spherical ERI tensor, C order eri[i,j,k,l]; small systems. */
+void orc_int2e_full(double *eri, const int *ao_loc, const int *atm, int natm,
+                    const int *bas, int nbas, const double *env)
+{
+        ensure_init();
+        size_t nao = ao_loc[nbas];
+        int maxd = 0, i;
+        for (i = 0; i < nbas; i++)
+                if (ao_loc[i + 1] - ao_loc[i] > maxd) maxd = ao_loc[i + 1] - ao_loc[i];
+#pragma omp parallel
+{
+        double *buf = malloc(sizeof(double) * maxd * maxd * maxd * maxd);
+        int ish, jsh, ksh, lsh;
+#pragma omp for schedule(dynamic) collapse(2)
+        for (ish = 0; ish < nbas; ish++)
+        for (jsh = 0; jsh < nbas; jsh++) {
+                for (ksh = 0; ksh < nbas; ksh++)
+                for (lsh = 0; lsh < nbas; lsh++) {
+                        int shls[4] = {ish, jsh, ksh, lsh};
+                        orc_int2e_sph(buf, shls, atm, natm, bas, nbas, env);
+                        int di = ao_loc[ish + 1] - ao_loc[ish], dj = ao_loc[jsh + 1] - ao_loc[jsh];
+                        int dk = ao_loc[ksh + 1] - ao_loc[ksh], dl = ao_loc[lsh + 1] - ao_loc[lsh];
+                        int a, b, c, d;
+                        for (a = 0; a < di; a++)
+                        for (b = 0; b < dj; b++)
+                        for (c = 0; c < dk; c++)
+                        for (d = 0; d < dl; d++)
+                                eri[(((ao_loc[ish] + a) * nao + ao_loc[jsh] + b) * nao
+                                     + ao_loc[ksh] + c) * nao + ao_loc[lsh] + d] =
+                                        buf[a + di * (b + dj * (c + (size_t)dk * d))];
+                }
+        }
+        free(buf);
+}
+}
+
+/* Schwarz bound q[I,J] = sqrt(max |(ij|ij)|) over AOs, floor 1e-100
+ * (lib/vhf/optimizer.c:408-454). */
+void orc_q_cond(double *q_cond, const int *ao_loc, const int *atm, int natm,
+                const int *bas, int nbas, const double *env)
+{
+        ensure_init();
+        int maxd = 0, i;
+        for (i = 0; i < nbas; i++)
+                if (ao_loc[i + 1] - ao_loc[i] > maxd) maxd = ao_loc[i + 1] - ao_loc[i];
+        size_t npair = (size_t)nbas * (nbas + 1) / 2;
+#pragma omp parallel
+{
+        double *buf = malloc(sizeof(double) * maxd * maxd * maxd * maxd);
+        size_t ij;
+#pragma omp for schedule(dynamic, 4)
+        for (ij = 0; ij < npair; ij++) {
+                int ish = (int)(sqrt(2. * ij + .25) - .5 + 1e-7);
+                int jsh = (int)(ij - (size_t)ish * (ish + 1) / 2);
+                int di = ao_loc[ish + 1] - ao_loc[ish], dj = ao_loc[jsh + 1] - ao_loc[jsh];
+                int shls[4] = {ish, jsh, ish, jsh};
+                orc_int2e_sph(buf, shls, atm, natm, bas, nbas, env);
+                double qmax = 1e-100;
+                int a, b;
+                for (a = 0; a < di; a++)
+                for (b = 0; b < dj; b++) {
+                        double v = fabs(buf[a + di * b + di * dj * a + di * dj * di * b]);
+                        if (v > qmax) qmax = v;
+                }
+                double q = (qmax > 1e-100) ? sqrt(qmax) : 1e-100;
+                q_cond[ish * nbas + jsh] = q;
+                q_cond[jsh * nbas + ish] = q;
+        }
+        free(buf);
+}
+}
+
+/* dm_cond[I,J] = 0.5 max_sets max_{i in I, j in J} (|D_ij| + |D_ji|)
+ * (lib/vhf/optimizer.c:494-518). */
+void orc_dm_cond(double *dm_cond, const double *dm, int nset, const int *ao_loc, int nbas)
+{
+        size_t nao = ao_loc[nbas];
+        int ish, jsh, iset, i, j;
+        for (ish = 0; ish < nbas; ish++)
+        for (jsh = 0; jsh <= ish; jsh++) {
+                double dmax = 0;
+                for (iset = 0; iset < nset; iset++) {
+                        const double *p = dm + nao * nao * iset;
+                        for (i = ao_loc[ish]; i < ao_loc[ish + 1]; i++)
+                        for (j = ao_loc[jsh]; j < ao_loc[jsh + 1]; j++) {
+                                double t = fabs(p[i * nao + j]) + fabs(p[j * nao + i]);
+                                if (t > dmax) dmax = t;
+                        }
+                }
+                dm_cond[ish * nbas + jsh] = .5 * dmax;
+                dm_cond[jsh * nbas + ish] = .5 * dmax;
+        }
+}
+
+/* Quartet predicate of CVHFnrs8_prescreen (lib/vhf/optimizer.c:90-117). */
+static int prescreen(int i, int j, int k, int l, const double *q, const double *dc, int n, double cut)
+{
+        double qijkl = q[i * n + j] * q[k * n + l];
+        return qijkl > cut
+            && ((4 * dc[j * n + i] * qijkl > cut)
+             || (4 * dc[l * n + k] * qijkl > cut)
+             || (dc[j * n + k] * qijkl > cut)
+             || (dc[j * n + l] * qijkl > cut)
+             || (dc[i * n + k] * qijkl > cut)
+             || (dc[i * n + l] * qijkl > cut));
+}
+
+/* Direct J/K over unique s8 shell quartets.  Outputs the FULL J (symmetric)
+ * and the FULL K (all 8 permutations) for each density; the caller applies
+ * the reference's lower-triangle/mirror convention.  q_cond == NULL means no
+ * screening (module-level get_jk without an optimizer).  If pair_stride > 1
+ * only bra pairs ij with ij % pair_stride == pair_offset are processed (a
+ * bounded sample for timing).  Returns the number of evaluated quartets. */
+long orc_direct_jk(const double *dms, int n_dm, double *vj, double *vk,
+                   int with_j, int with_k,
+                   const double *q_cond, const double *dm_cond, double cutoff,
+                   const int *ao_loc, const int *atm, int natm,
+                   const int *bas, int nbas, const double *env,
+                   int pair_stride, int pair_offset)
+{
+        ensure_init();
+        size_t nao = ao_loc[nbas];
+        size_t nn = nao * nao;
+        int maxd = 0, i;
+        for (i = 0; i < nbas; i++)
+                if (ao_loc[i + 1] - ao_loc[i] > maxd) maxd = ao_loc[i + 1] - ao_loc[i];
+        if (with_j) memset(vj, 0, sizeof(double) * nn * n_dm);
+        if (with_k) memset(vk, 0, sizeof(double) * nn * n_dm);
+        size_t npair = (size_t)nbas * (nbas + 1) / 2;
+        long nquartets = 0;
+        if (pair_stride < 1) pair_stride = 1;
+#pragma omp parallel reduction(+:nquartets)
+{
+        double *buf = malloc(sizeof(double) * maxd * maxd * maxd * maxd);
+        double *tj = with_j ? calloc(nn * n_dm, sizeof(double)) : NULL;
+        double *tk = with_k ? calloc(nn * n_dm, sizeof(double)) : NULL;
+        size_t ij;
+#pragma omp for schedule(dynamic, 1)
+        for (ij = 0; ij < npair; ij++) {
+                if ((long)(ij % pair_stride) != pair_offset) continue;
+                int ish = (int)(sqrt(2. * ij + .25) - .5 + 1e-7);
+                int jsh = (int)(ij - (size_t)ish * (ish + 1) / 2);
+                int ksh, lsh;
+                for (ksh = 0; ksh <= ish; ksh++)
+                for (lsh = 0; lsh <= ksh; lsh++) {
+                        if (ksh == ish && lsh > jsh) break;
+                        if (q_cond && !prescreen(ish, jsh, ksh, lsh, q_cond, dm_cond, nbas, cutoff))
+                                continue;
+                        int shls[4] = {ish, jsh, ksh, lsh};
+                        orc_int2e_sph(buf, shls, atm, natm, bas, nbas, env);
+                        nquartets++;
+                        double f = 1.;
+                        if (ish == jsh) f *= .5;
+                        if (ksh == lsh) f *= .5;
+                        if (ish == ksh && jsh == lsh) f *= .5;
+                        int i0 = ao_loc[ish], j0 = ao_loc[jsh], k0 = ao_loc[ksh], l0 = ao_loc[lsh];
+                        int di = ao_loc[ish + 1] - i0, dj = ao_loc[jsh + 1] - j0;
+                        int dk = ao_loc[ksh + 1] - k0, dl = ao_loc[lsh + 1] - l0;
+                        int a, b, c, d, s;
+                        for (d = 0; d < dl; d++)
+                        for (c = 0; c < dk; c++)
+                        for (b = 0; b < dj; b++)
+                        for (a = 0; a < di; a++) {
+                                double v = f * buf[a + di * (b + dj * (c + (size_t)dk * d))];
+                                size_t I = i0 + a, J = j0 + b, K = k0 + c, L = l0 + d;
+                                for (s = 0; s < n_dm; s++) {
+                                        const double *D = dms + nn * s;
+                                        if (with_j) {
+                                                double *V = tj + nn * s;
+                                                double dij = v * (D[I * nao + J] + D[J * nao + I]);
+                                                double dkl = v * (D[K * nao + L] + D[L * nao + K]);
+                                                V[K * nao + L] += dij;
+                                                V[L * nao + K] += dij;
+                                                V[I * nao + J] += dkl;
+                                                V[J * nao + I] += dkl;
+                                        }
+                                        if (with_k) {
+                                                double *V = tk + nn * s;
+                                                V[I * nao + L] += v * D[J * nao + K];
+                                                V[J * nao + L] += v * D[I * nao + K];
+                                                V[I * nao + K] += v * D[J * nao + L];
+                                                V[J * nao + K] += v * D[I * nao + L];
+                                                V[K * nao + J] += v * D[L * nao + I];
+                                                V[L * nao + J] += v * D[K * nao + I];
+                                                V[K * nao + I] += v * D[L * nao + J];
+                                                V[L * nao + I] += v * D[K * nao + J];
+                                        }
+                                }
+                        }
+                }
+        }
+#pragma omp critical
+        {
+                size_t x;
+                if (with_j) for (x = 0; x < nn * n_dm; x++) vj[x] += tj[x];
+                if (with_k) for (x = 0; x < nn * n_dm; x++) vk[x] += tk[x];
+        }
+        free(buf);
+        free(tj);
+        free(tk);
+}
+        return nquartets;
+}
+
+/* ---------------- one-electron integrals (for the oracle RHF) ---------------- */
+/* 1D overlap table S[a][b] for x^a x^b Gaussians about A, B (Obara-Saika). */
+static void overlap_1d(double S[H1 + 2][H1 + 2], int la, int lb, double p, double PA, double PB, double s00)
+{
+        int a, b;
+        double h = 0.5 / p;
+        S[0][0] = s00;
+        for (a = 0; a <= la; a++)
+        for (b = 0; b <= lb; b++) {
+                if (a == 0 && b == 0) continue;
+                double v;
+                if (a > 0) {
+                        v = PA * S[a - 1][b];
+                        if (a > 1) v += h * (a - 1) * S[a - 2][b];
+                        if (b > 0) v += h * b * S[a - 1][b - 1];
+                } else {
+                        v = PB * S[a][b - 1];
+                        if (b > 1) v += h * (b - 1) * S[a][b - 2];
+                }
+                S[a][b] = v;
+        }
+}
+
+/* which: 0 overlap, 1 kinetic, 2 nuclear attraction.  Output (nao, nao). */
+void orc_int1e(double *out, int which, const int *ao_loc, const int *atm, int natm,
+               const int *bas, int nbas, const double *env)
+{
+        ensure_init();
+        size_t nao = ao_loc[nbas];
+        int ish, jsh;
+        for (ish = 0; ish < nbas; ish++)
+        for (jsh = 0; jsh < nbas; jsh++) {
+                Shell si, sj;
+                get_shell(&si, ish, atm, bas, env);
+                get_shell(&sj, jsh, atm, bas, env);
+                int li = si.l, lj = sj.l, ni = ncart(li), nj = ncart(lj);
+                int Ni = ni * si.nctr, Nj = nj * sj.nctr;
+                double *cart = calloc((size_t)Ni * Nj, sizeof(double));
+                int ix[2][15], iy[2][15], iz[2][15], a, b, d;
+                const int ls[2] = {li, lj};
+                for (a = 0; a < 2; a++) {
+                        int n = 0, lx, ly;
+                        for (lx = ls[a]; lx >= 0; lx--)
+                                for (ly = ls[a] - lx; ly >= 0; ly--) {
+                                        ix[a][n] = lx; iy[a][n] = ly; iz[a][n] = ls[a] - lx - ly; n++;
+                                }
+                }
+                double AB2 = 0;
+                for (d = 0; d < 3; d++) AB2 += (si.r[d] - sj.r[d]) * (si.r[d] - sj.r[d]);
+                int pi, pj;
+                for (pi = 0; pi < si.nprim; pi++)
+                for (pj = 0; pj < sj.nprim; pj++) {
+                        double ai = si.exp[pi], aj = sj.exp[pj], p = ai + aj;
+                        double P[3];
+                        for (d = 0; d < 3; d++) P[d] = (ai * si.r[d] + aj * sj.r[d]) / p;
+                        double Kab = exp(-ai * aj / p * AB2);
+                        double prim[15 * 15];
+                        int fi, fj;
+                        if (which <= 1) {
+                                double S[3][H1 + 2][H1 + 2];
+                                for (d = 0; d < 3; d++)
+                                        overlap_1d(S[d], li + 1, lj + 2, p, P[d] - si.r[d], P[d] - sj.r[d],
+                                                   sqrt(M_PI / p) * (d == 0 ? Kab : 1.));
+                                for (fi = 0; fi < ni; fi++)
+                                for (fj = 0; fj < nj; fj++) {
+                                        int e[2][3] = {{ix[0][fi], iy[0][fi], iz[0][fi]},
+                                                       {ix[1][fj], iy[1][fj], iz[1][fj]}};
+                                        double v;
+                                        if (which == 0) {
+                                                v = S[0][e[0][0]][e[1][0]] * S[1][e[0][1]][e[1][1]] * S[2][e[0][2]][e[1][2]];
+                                        } else {
+                                                v = 0;
+                                                int dd;
+                                                for (dd = 0; dd < 3; dd++) {
+                                                        int A = e[0][dd], B = e[1][dd];
+                                                        double t = -2. * aj * (2 * B + 1) * S[dd][A][B]
+                                                                 + 4. * aj * aj * S[dd][A][B + 2];
+                                                        if (B >= 2) t += B * (B - 1) * S[dd][A][B - 2];
+                                                        t *= -0.5;
+                                                        double o = 1.;
+                                                        int d2;
+                                                        for (d2 = 0; d2 < 3; d2++)
+                                                                if (d2 != dd) o *= S[d2][e[0][d2]][e[1][d2]];
+                                                        v += t * o;
+                                                }
+                                        }
+                                        prim[fi * nj + fj] = v;
+                                }
+                        } else {
+                                memset(prim, 0, sizeof(prim));
+                                int ia, nroots = (li + lj) / 2 + 1, r;
+                                for (ia = 0; ia < natm; ia++) {
+                                        const double *C = env + atm[ia * ATM_SLOTS + PTR_COORD];
+                                        double Z = atm[ia * ATM_SLOTS + 0];
+                                        double PC2 = 0;
+                                        for (d = 0; d < 3; d++) PC2 += (P[d] - C[d]) * (P[d] - C[d]);
+                                        double xr[ORC_MAXROOTS], wr[ORC_MAXROOTS];
+                                        orc_rys_roots(nroots, p * PC2, xr, wr);
+                                        double fac = -Z * 2. * M_PI / p * Kab;
+                                        for (r = 0; r < nroots; r++) {
+                                                double t2 = xr[r];
+                                                double I1[3][H1][H1];
+                                                for (d = 0; d < 3; d++) {
+                                                        double g[G1];
+                                                        double C00 = P[d] - si.r[d] - t2 * (P[d] - C[d]);
+                                                        double B10 = 0.5 / p * (1. - t2);
+                                                        g[0] = (d == 2) ? wr[r] * fac : 1.;
+                                                        if (li + lj > 0) g[1] = C00 * g[0];
+                                                        for (a = 1; a < li + lj; a++)
+                                                                g[a + 1] = C00 * g[a] + a * B10 * g[a - 1];
+                                                        double h[G1][H1];
+                                                        for (a = 0; a <= li + lj; a++) h[a][0] = g[a];
+                                                        for (b = 1; b <= lj; b++)
+                                                                for (a = 0; a <= li + lj - b; a++)
+                                                                        h[a][b] = h[a + 1][b - 1] + (si.r[d] - sj.r[d]) * h[a][b - 1];
+                                                        for (a = 0; a <= li; a++)
+                                                                for (b = 0; b <= lj; b++) I1[d][a][b] = h[a][b];
+                                                }
+                                                for (fi = 0; fi < ni; fi++)
+                                                for (fj = 0; fj < nj; fj++)
+                                                        prim[fi * nj + fj] += I1[0][ix[0][fi]][ix[1][fj]]
+                                                                            * I1[1][iy[0][fi]][iy[1][fj]]
+                                                                            * I1[2][iz[0][fi]][iz[1][fj]];
+                                        }
+                                }
+                        }
+                        int ci, cj;
+                        for (ci = 0; ci < si.nctr; ci++)
+                        for (cj = 0; cj < sj.nctr; cj++) {
+                                double cc = si.coef[ci * si.nprim + pi] * sj.coef[cj * sj.nprim + pj];
+                                for (fi = 0; fi < ni; fi++)
+                                for (fj = 0; fj < nj; fj++)
+                                        cart[(ci * ni + fi) * Nj + cj * nj + fj] += cc * prim[fi * nj + fj];
+                        }
+                }
+                /* cart -> sph on both indices */
+                int ns_i = 2 * li + 1, ns_j = 2 * lj + 1, ci, cj, mi, mj, fi, fj;
+                for (ci = 0; ci < si.nctr; ci++)
+                for (cj = 0; cj < sj.nctr; cj++)
+                for (mi = 0; mi < ns_i; mi++)
+                for (mj = 0; mj < ns_j; mj++) {
+                        double s = 0;
+                        for (fi = 0; fi < ni; fi++) {
+                                double ci_ = c2s_tab[li][mi][fi];
+                                if (ci_ == 0) continue;
+                                for (fj = 0; fj < nj; fj++)
+                                        s += ci_ * c2s_tab[lj][mj][fj] * cart[(ci * ni + fi) * Nj + cj * nj + fj];
+                        }
+                        out[(ao_loc[ish] + ci * ns_i + mi) * nao + ao_loc[jsh] + cj * ns_j + mj] = s;
+                }
+                free(cart);
+        }
+}
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+        if (n > 0) omp_set_num_threads(n);
+#endif
+}

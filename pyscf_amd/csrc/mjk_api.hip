@@ -1,0 +1,952 @@
+// Host driver of the direct J/K engine: C ABI declared in include/mjk.h.
+//
+// Basis upload (once per molecule or batch):
+//   * every contracted shell with nctr > 1 is split into nctr segmented
+//     shells sharing the primitives, keeping PySCF's AO order (ctr, m);
+//   * shell pairs (i >= j) are built with the higher l first, primitive pairs
+//     with exp(-ab/(a+b)|AB|^2) below exp(-EXPCUTOFF) are dropped, and the
+//     prefactor sqrt(2) pi^(5/4) ca cb exp(-ab/p AB^2)/p is folded per pair;
+//   * Schwarz bounds are computed on the GPU (k_qcond), pairs are sorted by
+//     class and descending bound, and per quartet class a list of bra tiles
+//     with their Schwarz-surviving ket prefix is built.
+// J/K build: dm_cond -> sph->cart densities -> one kernel per quartet class
+// -> cart->sph J and K with the reference's symmetrisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/mjk.h"
+#include "c2s.h"
+#include "jk_common.h"
+#include "rys_eval.h"
+
+namespace mjk {
+int launch_jk(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, hipStream_t st);
+int launch_qcond(int la, int lb, const QCondArgs& a, hipStream_t st);
+int upload_c2s(hipStream_t st);
+int launch_sph2cart(const XformArgs& x, int maxld, const double* Dsph, long long sslot, int n_dm,
+                    int hermi, int with_j, int with_k, double* Dj, double* Dk, long long cslot,
+                    hipStream_t st);
+int launch_cart2sph(const XformArgs& x, int maxnao, const double* Jc, const double* Kc, long long cslot,
+                    int n_dm, int hermi, int with_j, int with_k, double* vj, double* vk, long long sslot,
+                    hipStream_t st);
+int launch_dmcond(const DmCondArgs& x, const double* Dsph, long long sslot, int n_dm, double* out,
+                  hipStream_t st);
+}  // namespace mjk
+
+using namespace mjk;
+
+namespace {
+
+constexpr double EXPCUTOFF = 60.0;   // drop primitive pairs with exp(-ab/p AB^2) < e^-60
+constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8, PTR_COORD = 1, ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2,
+              NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6;
+constexpr int NQCLASS_MAX = NPAIRCLASS * (NPAIRCLASS + 1) / 2;
+
+struct GShell {
+    int l, nprim, orig;
+    double r[3];
+    std::vector<double> e, c;
+    int s0, c0;   // molecule-local spherical / Cartesian offsets
+};
+
+struct MolHost {
+    int nao = 0, ncart = 0, nsh = 0, nbas_orig = 0;
+    std::vector<GShell> sh;
+    long long sbase = 0, dbase = 0, cbase = 0;
+    long long sph_first = 0, cart_first = 0, sh_first = 0;
+};
+
+struct QClass {
+    int X, Y, la, lb, lc, ld;
+    int nbt = 0;
+    long long nblocks = 0;
+    int4* bt = nullptr;
+    int* bt_mol = nullptr;
+    int* bt_off = nullptr;
+};
+
+int lmax_built()
+{
+    return 2;   // quartet kernels instantiated for l <= 2
+}
+
+double flops_class(int a, int b, int c, int d, double nq, double nprimq)
+{
+    const int L = a + b + c + d;
+    const double nr = L / 2 + 1;
+    const double Nc = (double)ncart(a) * ncart(b) * ncart(c) * ncart(d);
+    const double Ns = (double)nsph(a) * nsph(b) * nsph(c) * nsph(d);
+    const double fprim = 30 + 20 * nr + 12 * nr * (a + b + 1) * (c + d + 1) + 3 * nr * Nc + 2 * Nc;
+    double fc2s = 0;
+    if (a >= 2) fc2s += (double)nsph(a) * ncart(a) * ncart(b) * ncart(c) * ncart(d);
+    if (b >= 2) fc2s += (double)nsph(a) * nsph(b) * ncart(b) * ncart(c) * ncart(d);
+    if (c >= 2) fc2s += (double)nsph(a) * nsph(b) * nsph(c) * ncart(c) * ncart(d);
+    if (d >= 2) fc2s += (double)nsph(a) * nsph(b) * nsph(c) * nsph(d) * ncart(d);
+    fc2s *= 2;
+    const double fdig = 12 * Ns;
+    return nprimq * fprim + nq * (fc2s + fdig);
+}
+
+}  // namespace
+
+struct mjk_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0, used = 0;
+    std::string err;
+    double* rys = nullptr;
+    std::map<void*, size_t> allocs;
+    std::recursive_mutex mu;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+int fail(mjk_ctx* ctx, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                                            \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) return fail(ctx, MJK_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+int dalloc(mjk_ctx* ctx, size_t bytes, void** p)
+{
+    *p = nullptr;
+    if (bytes == 0) return MJK_OK;
+    if (ctx->used + bytes > ctx->budget)
+        return fail(ctx, MJK_ENOMEM, "HBM budget exceeded: need %zu more bytes, %zu of %zu in use", bytes,
+                    ctx->used, ctx->budget);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return fail(ctx, MJK_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    ctx->used += bytes;
+    ctx->allocs[*p] = bytes;
+    return MJK_OK;
+}
+
+void dfree(mjk_ctx* ctx, void* p)
+{
+    if (!p) return;
+    auto it = ctx->allocs.find(p);
+    if (it != ctx->allocs.end()) {
+        ctx->used -= it->second;
+        ctx->allocs.erase(it);
+    }
+    (void)hipFree(p);
+}
+
+template <typename T>
+int upload(mjk_ctx* ctx, const std::vector<T>& v, T** d)
+{
+    int rc = dalloc(ctx, v.size() * sizeof(T), (void**)d);
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(ctx, hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MJK_OK;
+}
+
+}  // namespace
+
+struct mjk_basis {
+    mjk_ctx* ctx = nullptr;
+    int nmol = 0;
+    std::vector<MolHost> mols;
+    int maxld = 0, maxnao = 0, maxnsh = 0;
+    long long slot_s = 0, slot_c = 0, cond_total = 0;
+    long long ncart_tot = 0, nsph_tot = 0, nsh_tot = 0;
+    // pairs (device)
+    int npairs = 0;
+    double4* ppP = nullptr;
+    double2* ppK = nullptr;
+    int *prim0 = nullptr, *nprim = nullptr, *shA = nullptr, *shB = nullptr, *cartA = nullptr, *cartB = nullptr;
+    double4 *rA = nullptr, *rAB = nullptr;
+    std::vector<int> seg;            // [class * nmol + mol] -> first pair; size NPAIRCLASS*nmol + 1
+    std::vector<double> q;           // Schwarz bound per pair (sorted order)
+    std::vector<int> h_shA, h_shB, h_nprim;
+    // per molecule / per AO device arrays
+    long long *d_dbase = nullptr, *d_cbase = nullptr, *d_sbase = nullptr, *d_cart_first = nullptr,
+              *d_sph_first = nullptr, *d_sh_first = nullptr;
+    int *d_ld = nullptr, *d_nsh = nullptr, *d_nao = nullptr;
+    int *d_cart_mol = nullptr, *d_cart_s0 = nullptr, *d_cart_l = nullptr, *d_cart_c = nullptr;
+    int *d_sph_mol = nullptr, *d_sph_c0 = nullptr, *d_sph_l = nullptr, *d_sph_m = nullptr;
+    int *d_sh_mol = nullptr, *d_sh_s0 = nullptr, *d_sh_n = nullptr;
+    // tiles
+    double tile_cutoff = -1.0;
+    std::vector<QClass> qcls;
+    // work buffers
+    double *Dsph = nullptr, *Vj = nullptr, *Vk = nullptr;
+    double *Dj = nullptr, *Dk = nullptr, *J = nullptr, *K = nullptr, *dmcond = nullptr;
+    long long cap_ds = 0, cap_vj = 0, cap_vk = 0, cap_dj = 0, cap_j = 0, cap_dk = 0, cap_k = 0, cap_cond = 0;
+    unsigned long long* counters = nullptr;
+    std::vector<void*> owned;
+};
+
+namespace {
+
+void free_tiles(mjk_basis* b)
+{
+    for (auto& c : b->qcls) {
+        dfree(b->ctx, c.bt);
+        dfree(b->ctx, c.bt_mol);
+        dfree(b->ctx, c.bt_off);
+    }
+    b->qcls.clear();
+    b->tile_cutoff = -1.0;
+}
+
+// Bra tiles of TILE pairs; each keeps the ket prefix whose Schwarz product
+// with the tile's largest bound survives the cutoff.
+int build_tiles(mjk_basis* b, double cutoff)
+{
+    mjk_ctx* ctx = b->ctx;
+    free_tiles(b);
+    const int nmol = b->nmol;
+    for (int X = 0; X < NPAIRCLASS; ++X) {
+        for (int Y = 0; Y <= X; ++Y) {
+            QClass qc;
+            qc.X = X;
+            qc.Y = Y;
+            // decode (la, lb) of a pair class
+            auto dec = [](int c, int& la, int& lb) {
+                la = 0;
+                while ((la + 1) * (la + 2) / 2 <= c) ++la;
+                lb = c - la * (la + 1) / 2;
+            };
+            dec(X, qc.la, qc.lb);
+            dec(Y, qc.lc, qc.ld);
+            std::vector<int4> bt;
+            std::vector<int> btm, off(1, 0);
+            long long total = 0;
+            for (int m = 0; m < nmol; ++m) {
+                const int x0 = b->seg[X * nmol + m], x1 = b->seg[X * nmol + m + 1];
+                const int y0 = b->seg[Y * nmol + m], y1 = b->seg[Y * nmol + m + 1];
+                if (x1 <= x0 || y1 <= y0) continue;
+                for (int p0 = x0; p0 < x1; p0 += TILE) {
+                    const int p1 = std::min(p0 + TILE, x1);
+                    int qend = y1;
+                    if (cutoff > 0) {
+                        const double qb = b->q[p0];
+                        // first ket index with q[y] * qb <= cutoff (q sorted descending)
+                        int lo = y0, hi = y1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) / 2;
+                            if (b->q[mid] * qb > cutoff) lo = mid + 1; else hi = mid;
+                        }
+                        qend = lo;
+                    }
+                    if (X == Y) qend = std::min(qend, p1);
+                    const int nkt = (qend - y0 + TILE - 1) / TILE;
+                    if (nkt <= 0) continue;
+                    bt.push_back(make_int4(p0, p1, y0, qend));
+                    btm.push_back(m);
+                    total += nkt;
+                    if (total > 2000000000LL)
+                        return fail(ctx, MJK_EUNSUPPORTED, "quartet class has more than 2^31 tiles");
+                    off.push_back((int)total);
+                }
+            }
+            qc.nbt = (int)bt.size();
+            qc.nblocks = total;
+            if (qc.nbt > 0) {
+                int rc;
+                if ((rc = upload(ctx, bt, &qc.bt))) return rc;
+                if ((rc = upload(ctx, btm, &qc.bt_mol))) return rc;
+                if ((rc = upload(ctx, off, &qc.bt_off))) return rc;
+            }
+            b->qcls.push_back(qc);
+        }
+    }
+    b->tile_cutoff = cutoff;
+    return MJK_OK;
+}
+
+PairSet pairset(const mjk_basis* b)
+{
+    PairSet P;
+    P.ppP = b->ppP;
+    P.ppK = b->ppK;
+    P.prim0 = b->prim0;
+    P.nprim = b->nprim;
+    P.rA = b->rA;
+    P.rAB = b->rAB;
+    P.shA = b->shA;
+    P.shB = b->shB;
+    P.cartA = b->cartA;
+    P.cartB = b->cartB;
+    return P;
+}
+
+XformArgs xform(const mjk_basis* b)
+{
+    XformArgs x;
+    x.cart_mol = b->d_cart_mol;
+    x.cart_s0 = b->d_cart_s0;
+    x.cart_l = b->d_cart_l;
+    x.cart_c = b->d_cart_c;
+    x.sph_mol = b->d_sph_mol;
+    x.sph_c0 = b->d_sph_c0;
+    x.sph_l = b->d_sph_l;
+    x.sph_m = b->d_sph_m;
+    x.dbase = b->d_dbase;
+    x.ld = b->d_ld;
+    x.sbase = b->d_sbase;
+    x.nao = b->d_nao;
+    x.cart_first = b->d_cart_first;
+    x.sph_first = b->d_sph_first;
+    x.nmol = b->nmol;
+    x.ncart_tot = b->ncart_tot;
+    x.nsph_tot = b->nsph_tot;
+    return x;
+}
+
+DmCondArgs dmcond_args(const mjk_basis* b)
+{
+    DmCondArgs d;
+    d.sh_mol = b->d_sh_mol;
+    d.sh_s0 = b->d_sh_s0;
+    d.sh_n = b->d_sh_n;
+    d.sh_first = b->d_sh_first;
+    d.cbase = b->d_cbase;
+    d.nsh = b->d_nsh;
+    d.sbase = b->d_sbase;
+    d.nao = b->d_nao;
+    d.nsh_tot = b->nsh_tot;
+    d.maxnsh = b->maxnsh;
+    return d;
+}
+
+template <typename T>
+int track_upload(mjk_basis* b, const std::vector<T>& v, T** d)
+{
+    int rc = upload(b->ctx, v, d);
+    if (rc == MJK_OK && *d) b->owned.push_back((void*)*d);
+    return rc;
+}
+
+int parse_molecule(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas, const double* env,
+                   MolHost& m)
+{
+    if (natm <= 0 || nbas <= 0 || !atm || !bas || !env) return fail(ctx, MJK_EINVAL, "empty molecule");
+    m.nbas_orig = nbas;
+    int s0 = 0, c0 = 0;
+    for (int i = 0; i < nbas; ++i) {
+        const int* r = bas + i * BAS_SLOTS;
+        const int l = r[ANG_OF], np = r[NPRIM_OF], nc = r[NCTR_OF], ia = r[ATOM_OF];
+        if (l < 0 || l > LMAX) return fail(ctx, MJK_EUNSUPPORTED, "shell %d has l=%d (max %d)", i, l, LMAX);
+        if (np <= 0 || nc <= 0 || ia < 0 || ia >= natm) return fail(ctx, MJK_EINVAL, "bad shell %d", i);
+        const double* xyz = env + atm[ia * ATM_SLOTS + PTR_COORD];
+        const double ang = (l == 0) ? C2S_S0 : (l == 1) ? C2S_P1 : 1.0;
+        for (int c = 0; c < nc; ++c) {
+            GShell g;
+            g.l = l;
+            g.nprim = np;
+            g.orig = i;
+            g.r[0] = xyz[0];
+            g.r[1] = xyz[1];
+            g.r[2] = xyz[2];
+            g.e.assign(env + r[PTR_EXP], env + r[PTR_EXP] + np);
+            g.c.resize(np);
+            for (int k = 0; k < np; ++k) g.c[k] = env[r[PTR_COEFF] + c * np + k] * ang;
+            g.s0 = s0;
+            g.c0 = c0;
+            s0 += nsph(l);
+            c0 += ncart(l);
+            m.sh.push_back(g);
+        }
+    }
+    m.nao = s0;
+    m.ncart = c0;
+    m.nsh = (int)m.sh.size();
+    return MJK_OK;
+}
+
+struct PairHost {
+    int A, B;          // molecule-local shell indices, l(A) >= l(B)
+    int mol;
+    std::vector<double4> P;
+    std::vector<double2> K;
+};
+
+int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
+{
+    const int nmol = b->nmol;
+    // offsets
+    long long sb = 0, cb = 0, kb = 0, sf = 0, cf = 0, hf = 0;
+    for (auto& m : b->mols) {
+        m.sbase = sb;
+        m.dbase = cb;
+        m.cbase = kb;
+        m.sph_first = sf;
+        m.cart_first = cf;
+        m.sh_first = hf;
+        sb += (long long)m.nao * m.nao;
+        cb += (long long)m.ncart * m.ncart;
+        kb += (long long)m.nsh * m.nsh;
+        sf += m.nao;
+        cf += m.ncart;
+        hf += m.nsh;
+        b->maxld = std::max(b->maxld, m.ncart);
+        b->maxnao = std::max(b->maxnao, m.nao);
+        b->maxnsh = std::max(b->maxnsh, m.nsh);
+    }
+    b->slot_s = sb;
+    b->slot_c = cb;
+    b->cond_total = kb;
+    b->nsph_tot = sf;
+    b->ncart_tot = cf;
+    b->nsh_tot = hf;
+
+    // shell pairs, grouped by class then molecule
+    std::vector<std::vector<PairHost>> byseg((size_t)NPAIRCLASS * nmol);
+    const double sq2pi54 = std::sqrt(2.0) * std::pow(M_PI, 1.25);
+    for (int mi = 0; mi < nmol; ++mi) {
+        const MolHost& m = b->mols[mi];
+        for (int i = 0; i < m.nsh; ++i) {
+            for (int j = 0; j <= i; ++j) {
+                int A = i, B = j;
+                if (m.sh[A].l < m.sh[B].l) std::swap(A, B);
+                const GShell& sa = m.sh[A];
+                const GShell& sbb = m.sh[B];
+                const double ABx = sa.r[0] - sbb.r[0], ABy = sa.r[1] - sbb.r[1], ABz = sa.r[2] - sbb.r[2];
+                const double rr = ABx * ABx + ABy * ABy + ABz * ABz;
+                PairHost ph;
+                ph.A = A;
+                ph.B = B;
+                ph.mol = mi;
+                for (int ia = 0; ia < sa.nprim; ++ia)
+                    for (int ib = 0; ib < sbb.nprim; ++ib) {
+                        const double ea = sa.e[ia], eb = sbb.e[ib];
+                        const double p = ea + eb;
+                        const double E = ea * eb / p * rr;
+                        if (E > EXPCUTOFF) continue;
+                        const double cc = sa.c[ia] * sbb.c[ib];
+                        if (cc == 0.0) continue;
+                        double4 P;
+                        P.x = (ea * sa.r[0] + eb * sbb.r[0]) / p;
+                        P.y = (ea * sa.r[1] + eb * sbb.r[1]) / p;
+                        P.z = (ea * sa.r[2] + eb * sbb.r[2]) / p;
+                        P.w = p;
+                        ph.P.push_back(P);
+                        ph.K.push_back(make_double2(cc * std::exp(-E) * sq2pi54 / p, 1.0 / p));
+                    }
+                if (ph.P.empty()) continue;
+                byseg[(size_t)pair_class(sa.l, sbb.l) * nmol + mi].push_back(std::move(ph));
+            }
+        }
+    }
+    // flatten (unsorted) for the Schwarz kernel
+    auto flatten = [&](std::vector<int>& order_seg_start) {
+        order_seg_start.assign((size_t)NPAIRCLASS * nmol + 1, 0);
+        int n = 0;
+        for (size_t s = 0; s < byseg.size(); ++s) {
+            order_seg_start[s] = n;
+            n += (int)byseg[s].size();
+        }
+        order_seg_start.back() = n;
+        return n;
+    };
+    auto upload_pairs = [&](std::vector<double>* qv) -> int {
+        std::vector<double4> hP, hA, hAB;
+        std::vector<double2> hK;
+        std::vector<int> hp0, hnp, hsa, hsb, hca, hcb;
+        for (auto& segv : byseg)
+            for (auto& ph : segv) {
+                const MolHost& m = b->mols[ph.mol];
+                const GShell& sa = m.sh[ph.A];
+                const GShell& sbb = m.sh[ph.B];
+                hp0.push_back((int)hP.size());
+                hnp.push_back((int)ph.P.size());
+                hP.insert(hP.end(), ph.P.begin(), ph.P.end());
+                hK.insert(hK.end(), ph.K.begin(), ph.K.end());
+                const double qq = qv ? (*qv)[hsa.size()] : 0.0;
+                hA.push_back(make_double4(sa.r[0], sa.r[1], sa.r[2], qq));
+                hAB.push_back(make_double4(sa.r[0] - sbb.r[0], sa.r[1] - sbb.r[1], sa.r[2] - sbb.r[2], 0.0));
+                hsa.push_back(ph.A);
+                hsb.push_back(ph.B);
+                hca.push_back(sa.c0);
+                hcb.push_back(sbb.c0);
+            }
+        if (hP.size() > (size_t)2000000000) return fail(ctx, MJK_EUNSUPPORTED, "too many primitive pairs");
+        int rc;
+        if ((rc = track_upload(b, hP, &b->ppP))) return rc;
+        if ((rc = track_upload(b, hK, &b->ppK))) return rc;
+        if ((rc = track_upload(b, hp0, &b->prim0))) return rc;
+        if ((rc = track_upload(b, hnp, &b->nprim))) return rc;
+        if ((rc = track_upload(b, hA, &b->rA))) return rc;
+        if ((rc = track_upload(b, hAB, &b->rAB))) return rc;
+        if ((rc = track_upload(b, hsa, &b->shA))) return rc;
+        if ((rc = track_upload(b, hsb, &b->shB))) return rc;
+        if ((rc = track_upload(b, hca, &b->cartA))) return rc;
+        if ((rc = track_upload(b, hcb, &b->cartB))) return rc;
+        b->h_shA = hsa;
+        b->h_shB = hsb;
+        b->h_nprim = hnp;
+        return MJK_OK;
+    };
+    auto release_pairs = [&]() {
+        void* ps[] = {b->ppP, b->ppK, b->prim0, b->nprim, b->rA, b->rAB, b->shA, b->shB, b->cartA, b->cartB};
+        for (void* p : ps) {
+            dfree(ctx, p);
+            b->owned.erase(std::remove(b->owned.begin(), b->owned.end(), p), b->owned.end());
+        }
+    };
+
+    std::vector<int> seg0;
+    const int np = flatten(seg0);
+    b->npairs = np;
+    int rc = upload_pairs(nullptr);
+    if (rc) return rc;
+    // Schwarz bounds on the device, one launch per pair class
+    double* dq = nullptr;
+    if ((rc = dalloc(ctx, sizeof(double) * std::max(np, 1), (void**)&dq))) return rc;
+    for (int X = 0; X < NPAIRCLASS; ++X) {
+        const int p0 = seg0[(size_t)X * nmol], p1 = seg0[(size_t)(X + 1) * nmol];
+        if (p1 <= p0) continue;
+        int la = 0;
+        while ((la + 1) * (la + 2) / 2 <= X) ++la;
+        const int lb = X - la * (la + 1) / 2;
+        if (la > lmax_built()) {
+            dfree(ctx, dq);
+            return fail(ctx, MJK_EUNSUPPORTED, "basis has l=%d shells; kernels are built for l<=%d", la,
+                        lmax_built());
+        }
+        QCondArgs qa;
+        qa.P = pairset(b);
+        qa.p0 = p0;
+        qa.np = p1 - p0;
+        qa.qout = dq;
+        qa.rys = ctx->rys;
+        if (launch_qcond(la, lb, qa, ctx->stream)) {
+            dfree(ctx, dq);
+            return fail(ctx, MJK_EDEVICE, "q_cond launch failed for class (%d,%d)", la, lb);
+        }
+    }
+    std::vector<double> qflat(np);
+    HIPCHK(ctx, hipMemcpyAsync(qflat.data(), dq, sizeof(double) * np, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    dfree(ctx, dq);
+    release_pairs();
+    // sort each (class, molecule) segment by descending bound
+    {
+        int k = 0;
+        for (auto& segv : byseg) {
+            std::vector<int> idx(segv.size());
+            std::iota(idx.begin(), idx.end(), 0);
+            std::vector<double> qs(qflat.begin() + k, qflat.begin() + k + segv.size());
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return qs[x] > qs[y]; });
+            std::vector<PairHost> sorted;
+            sorted.reserve(segv.size());
+            for (int i : idx) sorted.push_back(std::move(segv[i]));
+            segv = std::move(sorted);
+            for (size_t i = 0; i < idx.size(); ++i) qflat[k + i] = qs[idx[i]];
+            k += (int)segv.size();
+        }
+    }
+    b->q = qflat;
+    flatten(b->seg);
+    if ((rc = upload_pairs(&b->q))) return rc;
+
+    // per molecule / AO maps
+    std::vector<long long> dbase, cbase, sbase, cfirst, sfirst, hfirst;
+    std::vector<int> ld, nsh, nao, cmol, cs0, cl, cc, smol, sc0, sl, sm, hmol, hs0, hn;
+    for (int mi = 0; mi < nmol; ++mi) {
+        const MolHost& m = b->mols[mi];
+        dbase.push_back(m.dbase);
+        cbase.push_back(m.cbase);
+        sbase.push_back(m.sbase);
+        cfirst.push_back(m.cart_first);
+        sfirst.push_back(m.sph_first);
+        hfirst.push_back(m.sh_first);
+        ld.push_back(m.ncart);
+        nsh.push_back(m.nsh);
+        nao.push_back(m.nao);
+        for (const GShell& g : m.sh) {
+            for (int c = 0; c < ncart(g.l); ++c) {
+                cmol.push_back(mi);
+                cs0.push_back(g.s0);
+                cl.push_back(g.l);
+                cc.push_back(c);
+            }
+            for (int s = 0; s < nsph(g.l); ++s) {
+                smol.push_back(mi);
+                sc0.push_back(g.c0);
+                sl.push_back(g.l);
+                sm.push_back(s);
+            }
+            hmol.push_back(mi);
+            hs0.push_back(g.s0);
+            hn.push_back(nsph(g.l));
+        }
+    }
+    if ((rc = track_upload(b, dbase, &b->d_dbase))) return rc;
+    if ((rc = track_upload(b, cbase, &b->d_cbase))) return rc;
+    if ((rc = track_upload(b, sbase, &b->d_sbase))) return rc;
+    if ((rc = track_upload(b, cfirst, &b->d_cart_first))) return rc;
+    if ((rc = track_upload(b, sfirst, &b->d_sph_first))) return rc;
+    if ((rc = track_upload(b, hfirst, &b->d_sh_first))) return rc;
+    if ((rc = track_upload(b, ld, &b->d_ld))) return rc;
+    if ((rc = track_upload(b, nsh, &b->d_nsh))) return rc;
+    if ((rc = track_upload(b, nao, &b->d_nao))) return rc;
+    if ((rc = track_upload(b, cmol, &b->d_cart_mol))) return rc;
+    if ((rc = track_upload(b, cs0, &b->d_cart_s0))) return rc;
+    if ((rc = track_upload(b, cl, &b->d_cart_l))) return rc;
+    if ((rc = track_upload(b, cc, &b->d_cart_c))) return rc;
+    if ((rc = track_upload(b, smol, &b->d_sph_mol))) return rc;
+    if ((rc = track_upload(b, sc0, &b->d_sph_c0))) return rc;
+    if ((rc = track_upload(b, sl, &b->d_sph_l))) return rc;
+    if ((rc = track_upload(b, sm, &b->d_sph_m))) return rc;
+    if ((rc = track_upload(b, hmol, &b->d_sh_mol))) return rc;
+    if ((rc = track_upload(b, hs0, &b->d_sh_s0))) return rc;
+    if ((rc = track_upload(b, hn, &b->d_sh_n))) return rc;
+    if ((rc = dalloc(ctx, sizeof(unsigned long long) * 2 * NQCLASS_MAX, (void**)&b->counters))) return rc;
+    b->owned.push_back(b->counters);
+    return build_tiles(b, cutoff);
+}
+
+int ensure(mjk_ctx* ctx, double** p, long long& cap, long long need)
+{
+    if (need <= cap && *p) return MJK_OK;
+    dfree(ctx, *p);
+    *p = nullptr;
+    cap = 0;
+    int rc = dalloc(ctx, sizeof(double) * std::max(need, 1LL), (void**)p);
+    if (rc) return rc;
+    cap = need;
+    return MJK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
+{
+    if (!out) return MJK_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MJK_EDEVICE;
+    if (device < 0 || device >= ndev) return MJK_EINVAL;
+    mjk_ctx* ctx = new mjk_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MJK_EDEVICE;
+    }
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    ctx->budget = hbm_budget ? hbm_budget : (size_t)(0.8 * (double)fr);
+    for (auto& e : ctx->ev) (void)hipEventCreate(&e);
+    std::vector<double> tab(RYS_BUF_SIZE);
+    rys_fill_buffer(tab.data());
+    if (upload(ctx, tab, &ctx->rys) || upload_c2s(ctx->stream) || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        mjk_ctx_destroy(ctx);
+        return MJK_EDEVICE;
+    }
+    *out = ctx;
+    return MJK_OK;
+}
+
+void mjk_ctx_destroy(mjk_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
+    ctx->allocs.clear();
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* mjk_last_error(const mjk_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : "no context";
+}
+
+size_t mjk_ctx_hbm_used(const mjk_ctx* ctx)
+{
+    return ctx ? ctx->used : 0;
+}
+
+int mjk_basis_upload_batch(mjk_ctx* ctx, int nmol, const int* natm, const int* nbas, const int* const* atm,
+                           const int* const* bas, const double* const* env, double cutoff, mjk_basis** out)
+{
+    if (!ctx || !out || nmol <= 0) return fail(ctx, MJK_EINVAL, "bad arguments to mjk_basis_upload_batch");
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    *out = nullptr;
+    mjk_basis* b = new mjk_basis();
+    b->ctx = ctx;
+    b->nmol = nmol;
+    b->mols.resize(nmol);
+    int rc = MJK_OK;
+    for (int m = 0; m < nmol && rc == MJK_OK; ++m)
+        rc = parse_molecule(ctx, atm[m], natm[m], bas[m], nbas[m], env[m], b->mols[m]);
+    if (rc == MJK_OK) rc = build_basis(ctx, b, cutoff);
+    if (rc != MJK_OK) {
+        std::string keep = ctx->err;
+        mjk_basis_destroy(b);
+        ctx->err = keep;
+        return rc;
+    }
+    *out = b;
+    return MJK_OK;
+}
+
+int mjk_basis_upload(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas, const double* env,
+                     double cutoff, mjk_basis** out)
+{
+    return mjk_basis_upload_batch(ctx, 1, &natm, &nbas, &atm, &bas, &env, cutoff, out);
+}
+
+void mjk_basis_destroy(mjk_basis* b)
+{
+    if (!b) return;
+    mjk_ctx* ctx = b->ctx;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_tiles(b);
+    for (void* p : b->owned) dfree(ctx, p);
+    double* work[] = {b->Dsph, b->Vj, b->Vk, b->Dj, b->Dk, b->J, b->K, b->dmcond};
+    for (double* p : work) dfree(ctx, p);
+    delete b;
+}
+
+int mjk_basis_nao(const mjk_basis* b, int mol)
+{
+    if (!b || mol < 0 || mol >= b->nmol) return MJK_EINVAL;
+    return b->mols[mol].nao;
+}
+
+long long mjk_basis_slot_size(const mjk_basis* b)
+{
+    return b ? b->slot_s : MJK_EINVAL;
+}
+
+int mjk_basis_q_cond(mjk_ctx* ctx, const mjk_basis* b, int mol, double* q_cond)
+{
+    if (!ctx || !b || mol < 0 || mol >= b->nmol || !q_cond) return fail(ctx, MJK_EINVAL, "bad arguments");
+    const MolHost& m = b->mols[mol];
+    const int n = m.nbas_orig;
+    for (int i = 0; i < n * n; ++i) q_cond[i] = 1e-100;
+    for (int X = 0; X < NPAIRCLASS; ++X) {
+        const int p0 = b->seg[(size_t)X * b->nmol + mol], p1 = b->seg[(size_t)X * b->nmol + mol + 1];
+        for (int p = p0; p < p1; ++p) {
+            const int i = m.sh[b->h_shA[p]].orig, j = m.sh[b->h_shB[p]].orig;
+            q_cond[i * n + j] = std::max(q_cond[i * n + j], b->q[p]);
+            q_cond[j * n + i] = q_cond[i * n + j];
+        }
+    }
+    return MJK_OK;
+}
+
+int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm, int hermi, int with_j,
+                      int with_k, double direct_scf_tol, double fp32_tol, double* d_vj, double* d_vk,
+                      mjk_stats* stats)
+{
+    if (!ctx || !b || b->ctx != ctx) return fail(ctx, MJK_EINVAL, "context/basis mismatch");
+    if (n_dm <= 0 || !d_dms) return fail(ctx, MJK_EINVAL, "n_dm must be positive");
+    if (hermi < 0 || hermi > 2) return fail(ctx, MJK_EINVAL, "hermi must be 0, 1 or 2");
+    if ((with_j && !d_vj) || (with_k && !d_vk)) return fail(ctx, MJK_EINVAL, "missing output buffer");
+    if (fp32_tol > 0) return fail(ctx, MJK_EUNSUPPORTED, "mixed precision is not built yet");
+    if (!with_j && !with_k) return MJK_OK;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const bool screen = direct_scf_tol > 0;
+    const double cut = screen ? direct_scf_tol : 0.0;
+    int rc;
+    if (b->tile_cutoff != cut)
+        if ((rc = build_tiles(b, cut))) return rc;
+    const int n_j = with_j ? n_dm : 0;
+    const int n_k = with_k ? (hermi == 1 ? n_dm : 2 * n_dm) : 0;
+    if ((rc = ensure(ctx, &b->Dj, b->cap_dj, (long long)std::max(n_j, 1) * b->slot_c))) return rc;
+    if ((rc = ensure(ctx, &b->J, b->cap_j, (long long)std::max(n_j, 1) * b->slot_c))) return rc;
+    if ((rc = ensure(ctx, &b->Dk, b->cap_dk, (long long)std::max(n_k, 1) * b->slot_c))) return rc;
+    if ((rc = ensure(ctx, &b->K, b->cap_k, (long long)std::max(n_k, 1) * b->slot_c))) return rc;
+    if (screen && (rc = ensure(ctx, &b->dmcond, b->cap_cond, b->cond_total))) return rc;
+    hipStream_t st = ctx->stream;
+    HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
+    if (n_j) HIPCHK(ctx, hipMemsetAsync(b->J, 0, sizeof(double) * n_j * b->slot_c, st));
+    if (n_k) HIPCHK(ctx, hipMemsetAsync(b->K, 0, sizeof(double) * n_k * b->slot_c, st));
+    HIPCHK(ctx, hipMemsetAsync(b->counters, 0, sizeof(unsigned long long) * 2 * NQCLASS_MAX, st));
+    if (screen && launch_dmcond(dmcond_args(b), d_dms, b->slot_s, n_dm, b->dmcond, st))
+        return fail(ctx, MJK_EDEVICE, "dm_cond launch failed");
+    if (launch_sph2cart(xform(b), b->maxld, d_dms, b->slot_s, n_dm, hermi, with_j, with_k, b->Dj, b->Dk,
+                        b->slot_c, st))
+        return fail(ctx, MJK_EDEVICE, "density transform launch failed");
+    HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
+    JKArgs a;
+    a.P = pairset(b);
+    a.M.dbase = b->d_dbase;
+    a.M.ld = b->d_ld;
+    a.M.cbase = b->d_cbase;
+    a.M.nsh = b->d_nsh;
+    a.Dj = b->Dj;
+    a.Dk = b->Dk;
+    a.J = b->J;
+    a.K = b->K;
+    a.slot = b->slot_c;
+    a.n_j = n_j;
+    a.n_k = n_k;
+    a.dm_cond = b->dmcond;
+    a.cutoff = cut;
+    a.screen = screen ? 1 : 0;
+    a.rys = ctx->rys;
+    for (size_t c = 0; c < b->qcls.size(); ++c) {
+        const QClass& qc = b->qcls[c];
+        if (qc.nblocks == 0) continue;
+        a.bt = qc.bt;
+        a.bt_mol = qc.bt_mol;
+        a.bt_off = qc.bt_off;
+        a.nbt = qc.nbt;
+        a.diag = qc.X == qc.Y;
+        a.counters = b->counters + 2 * c;
+        if (launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
+            return fail(ctx, MJK_EUNSUPPORTED, "no kernel for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
+    if (launch_cart2sph(xform(b), b->maxnao, b->J, b->K, b->slot_c, n_dm, hermi, with_j, with_k, d_vj, d_vk,
+                        b->slot_s, st))
+        return fail(ctx, MJK_EDEVICE, "output transform launch failed");
+    HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
+    if (stats) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        float t01 = 0, t12 = 0, t23 = 0;
+        HIPCHK(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+        HIPCHK(ctx, hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+        HIPCHK(ctx, hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
+        std::vector<unsigned long long> cnt(2 * NQCLASS_MAX);
+        HIPCHK(ctx, hipMemcpy(cnt.data(), b->counters, sizeof(unsigned long long) * cnt.size(),
+                              hipMemcpyDeviceToHost));
+        stats->ms_quartets = t12;
+        stats->ms_transform = t01 + t23;
+        stats->quartets = 0;
+        stats->prim_quartets = 0;
+        stats->flops = 0;
+        for (size_t c = 0; c < b->qcls.size(); ++c) {
+            const QClass& qc = b->qcls[c];
+            stats->quartets += (long long)cnt[2 * c];
+            stats->prim_quartets += (long long)cnt[2 * c + 1];
+            stats->flops += flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)cnt[2 * c], (double)cnt[2 * c + 1]);
+        }
+        stats->ms_total =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return MJK_OK;
+}
+
+int mjk_get_jk(mjk_ctx* ctx, mjk_basis* b, const double* dms, int n_dm, int hermi, int with_j, int with_k,
+               double direct_scf_tol, double fp32_tol, double* vj, double* vk, mjk_stats* stats)
+{
+    if (!ctx || !b || !dms || n_dm <= 0) return fail(ctx, MJK_EINVAL, "bad arguments to mjk_get_jk");
+    auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    std::lock_guard<std::recursive_mutex> outer(ctx->mu);
+    {
+        HIPCHK(ctx, hipSetDevice(ctx->device));
+        const long long n = (long long)n_dm * b->slot_s;
+        if ((rc = ensure(ctx, &b->Dsph, b->cap_ds, n))) return rc;
+        if (with_j && (rc = ensure(ctx, &b->Vj, b->cap_vj, n))) return rc;
+        if (with_k && (rc = ensure(ctx, &b->Vk, b->cap_vk, n))) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(b->Dsph, dms, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    }
+    rc = mjk_get_jk_device(ctx, b, b->Dsph, n_dm, hermi, with_j, with_k, direct_scf_tol, fp32_tol,
+                           with_j ? b->Vj : nullptr, with_k ? b->Vk : nullptr, stats);
+    if (rc) return rc;
+    const long long n = (long long)n_dm * b->slot_s;
+    if (with_j) HIPCHK(ctx, hipMemcpyAsync(vj, b->Vj, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (with_k) HIPCHK(ctx, hipMemcpyAsync(vk, b->Vk, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (stats)
+        stats->ms_total =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return MJK_OK;
+}
+
+int mjk_device_alloc(mjk_ctx* ctx, size_t bytes, void** ptr)
+{
+    if (!ctx || !ptr) return MJK_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return dalloc(ctx, bytes, ptr);
+}
+
+int mjk_device_free(mjk_ctx* ctx, void* ptr)
+{
+    if (!ctx) return MJK_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    dfree(ctx, ptr);
+    return MJK_OK;
+}
+
+int mjk_memcpy_h2d(mjk_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return MJK_EINVAL;
+    HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MJK_OK;
+}
+
+int mjk_memcpy_d2h(mjk_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx) return MJK_EINVAL;
+    HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MJK_OK;
+}
+
+int mjk_synchronize(mjk_ctx* ctx)
+{
+    if (!ctx) return MJK_EINVAL;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MJK_OK;
+}
+
+int mjk_rys_roots(int n, double T, double* x, double* w)
+{
+    if (n < 1 || n > RYS_NMAX || !(T >= 0) || !x || !w) return MJK_EINVAL;
+    static std::vector<double> tab;
+    if (tab.empty()) {
+        tab.resize(RYS_BUF_SIZE);
+        rys_fill_buffer(tab.data());
+    }
+    switch (n) {
+#define R(N)                                          \
+    case N: {                                         \
+        double xx[N], ww[N];                          \
+        rys_roots<N>(tab.data(), T, xx, ww);          \
+        for (int i = 0; i < N; ++i) {                 \
+            x[i] = xx[i];                             \
+            w[i] = ww[i];                             \
+        }                                             \
+        return MJK_OK;                                \
+    }
+        R(1) R(2) R(3) R(4) R(5) R(6) R(7) R(8) R(9)
+#undef R
+    }
+    return MJK_EINVAL;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// Rys roots and weights from the interpolation table of gen_rys_table.py.
+// Usable on host (unit tests through the C-ABI) and device (kernels).
+//
+// For n roots and T < 36 + 6n: interval iv = floor(T/2), local variable
+// s = T - 2 iv - 1 in [-1, 1], each root x_i (= t_i^2) and weight w_i is a
+// 14-term Chebyshev series evaluated by Clenshaw's recurrence.  For larger T
+// the half-range Gauss-Hermite limit x_i = s_i^2/T, w_i = W_i/sqrt(T).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#define RYS_STORAGE static constexpr
+#include "rys_table.inc"
+#undef RYS_STORAGE
+
+namespace mjk {
+
+// Offset of the Hermite constants behind the Chebyshev coefficients in the
+// single device buffer holding both.
+constexpr int RYS_HERM_BASE = RYS_TABLE_SIZE;
+constexpr int RYS_BUF_SIZE = RYS_TABLE_SIZE + RYS_HOFF[RYS_NMAX + 1];
+
+__host__ __device__ __forceinline__ double cheb14(const double* __restrict__ c, double s)
+{
+    const double s2 = s + s;
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int k = RYS_NCOEF - 1; k >= 1; --k) {
+        const double t = fma(s2, b1, c[k] - b2);
+        b2 = b1;
+        b1 = t;
+    }
+    return fma(s, b1, c[0] - b2);
+}
+
+// tab: RYS_COEF followed by RYS_HERM (RYS_BUF_SIZE doubles).
+template <int N>
+__host__ __device__ __forceinline__ void rys_roots(const double* __restrict__ tab, double T,
+                                                   double (&x)[N], double (&w)[N])
+{
+    if (!(T < 36.0 + 6.0 * N)) {   // also catches NaN: constant indices only
+        const double it = 1.0 / T;
+        const double st = sqrt(it);
+        const double* h = tab + RYS_HERM_BASE + RYS_HOFF[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            x[i] = h[2 * i] * it;
+            w[i] = h[2 * i + 1] * st;
+        }
+    } else {
+        int iv = (int)(T * 0.5);
+        iv = iv < 0 ? 0 : (iv > 17 + 3 * N ? 17 + 3 * N : iv);
+        const double s = T - 2.0 * iv - 1.0;
+        const double* c = tab + RYS_OFF[N] + iv * (N * 2 * RYS_NCOEF);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            x[i] = cheb14(c + (2 * i) * RYS_NCOEF, s);
+            w[i] = cheb14(c + (2 * i + 1) * RYS_NCOEF, s);
+        }
+    }
+}
+
+// Root-by-root evaluation: prepare once per T, then root(i) gives (x_i, w_i).
+// Lets kernels keep a rolled root loop (one root's working set live at a time).
+template <int N>
+struct RysCursor {
+    const double* c;   // interval coefficients, or Hermite constants
+    double s;          // local variable (series) or 1/T (asymptotic)
+    double st;         // 1/sqrt(T) (asymptotic)
+    bool asym;
+    __host__ __device__ __forceinline__ RysCursor(const double* __restrict__ tab, double T)
+    {
+        asym = !(T < 36.0 + 6.0 * N);   // also catches NaN: constant indices only
+        if (asym) {
+            s = 1.0 / T;
+            st = sqrt(s);
+            c = tab + RYS_HERM_BASE + RYS_HOFF[N];
+        } else {
+            int iv = (int)(T * 0.5);
+            iv = iv < 0 ? 0 : (iv > 17 + 3 * N ? 17 + 3 * N : iv);
+            s = T - 2.0 * iv - 1.0;
+            st = 0.0;
+            c = tab + RYS_OFF[N] + iv * (N * 2 * RYS_NCOEF);
+        }
+    }
+    __host__ __device__ __forceinline__ void root(int i, double& x, double& w) const
+    {
+        if (asym) {
+            x = c[2 * i] * s;
+            w = c[2 * i + 1] * st;
+        } else {
+            x = cheb14(c + (2 * i) * RYS_NCOEF, s);
+            w = cheb14(c + (2 * i + 1) * RYS_NCOEF, s);
+        }
+    }
+};
+
+// Host-side helper: the table as one flat array (coefficients then Hermite).
+inline void rys_fill_buffer(double* buf)
+{
+    for (int i = 0; i < RYS_TABLE_SIZE; ++i) buf[i] = RYS_COEF[i];
+    for (int i = 0; i < RYS_HOFF[RYS_NMAX + 1]; ++i) buf[RYS_HERM_BASE + i] = RYS_HERM[i];
+}
+
+}  // namespace mjk

@@ -1,0 +1,283 @@
+"""Molecule and basis model ("Mole-lite") feeding the direct J/K engine.
+
+Produces the same ``_atm`` / ``_bas`` / ``_env`` integer/double arrays as
+PySCF's ``Mole.build`` so that the C-ABI consumes either object unchanged.
+
+Reference behaviour restated here:
+  * slot layout of ``_atm``/``_bas``/``_env``    pyscf/gto/mole.py:58-88
+  * ``gto_norm`` (radial normalisation)          pyscf/gto/mole.py:125-155
+  * ``make_atm_env`` / ``make_bas_env``          pyscf/gto/mole.py:965-1020
+  * contraction normalisation                    pyscf/gto/mole.py:1022-1031
+  * ``make_env`` (atoms first, then one basis
+    block per element, then per-atom bas rows)   pyscf/gto/mole.py:1033-1112
+  * Angstrom -> Bohr with BOHR = 0.52917721092   pyscf/data/nist.py:24
+  * ``energy_nuc``                               pyscf/gto/mole.py:1526-1551
+  * ``ao_loc_nr`` = cumsum((2l+1)*nctr)          pyscf/gto/moleintor.py:804-820
+"""
+
+import math
+import numpy
+
+from pyscf_amd.gto import basis as basis_mod
+
+BOHR = 0.52917721092          # Angstrom per Bohr (pyscf/data/nist.py:24)
+
+# _atm / _bas / _env slots (pyscf/gto/mole.py:58-88)
+CHARGE_OF = 0
+PTR_COORD = 1
+NUC_MOD_OF = 2
+PTR_ZETA = 3
+PTR_FRAC_CHARGE = 4
+PTR_RADIUS = 5
+ATM_SLOTS = 6
+ATOM_OF = 0
+ANG_OF = 1
+NPRIM_OF = 2
+NCTR_OF = 3
+KAPPA_OF = 4
+PTR_EXP = 5
+PTR_COEFF = 6
+BAS_SLOTS = 8
+PTR_EXPCUTOFF = 0
+PTR_RANGE_OMEGA = 8
+PTR_ENV_START = 20
+NUC_POINT = 1
+
+ELEMENTS = ['X', 'H', 'He', 'Li', 'Be', 'B', 'C', 'N', 'O', 'F', 'Ne',
+            'Na', 'Mg', 'Al', 'Si', 'P', 'S', 'Cl', 'Ar']
+_Z = {s.upper(): z for z, s in enumerate(ELEMENTS)}
+
+
+def std_symbol(symb):
+    """Element symbol with standard capitalisation ('o' -> 'O', 8 -> 'O')."""
+    if isinstance(symb, (int, numpy.integer)):
+        return ELEMENTS[int(symb)]
+    s = ''.join(c for c in str(symb) if c.isalpha())
+    if s.upper() not in _Z:
+        raise KeyError('unknown element %r' % symb)
+    return ELEMENTS[_Z[s.upper()]]
+
+
+def charge(symb):
+    return _Z[std_symbol(symb).upper()]
+
+
+def gaussian_int(n, alpha):
+    r"""int_0^inf x^n exp(-alpha x^2) dx"""
+    n1 = (n + 1) * .5
+    return math.gamma(n1) / (2. * alpha**n1)
+
+
+def gto_norm(l, expnt):
+    """Radial normalisation 1/sqrt(int r^(2l+2) exp(-2 a r^2) dr)."""
+    expnt = numpy.asarray(expnt, dtype=float)
+    n1 = (2 * l + 3) * .5
+    return 1. / numpy.sqrt(math.gamma(n1) / (2. * (2 * expnt)**n1))
+
+
+def _normalize_contracted(l, es, cs):
+    ee = es[:, None] + es[None, :]
+    n1 = (2 * l + 3) * .5
+    ee = math.gamma(n1) / (2. * ee**n1)
+    s1 = 1. / numpy.sqrt(numpy.einsum('pi,pq,qi->i', cs, ee, cs))
+    return cs * s1
+
+
+def make_bas_env(shells, atom_id=0, ptr=0):
+    """Rows ``[atom, l, nprim, nctr, 0, ptr_exp, ptr_coeff, 0]`` and the env
+    block holding exps[nprim] then coeffs[nctr][nprim] for each shell.
+    Primitives are sorted by descending exponent and the radial and
+    contraction normalisations are folded into the coefficients."""
+    bas = []
+    env = []
+    for sh in shells:
+        l = sh[0]
+        rows = sorted((list(r) for r in sh[1:]), reverse=True)
+        arr = numpy.array(rows, dtype=float)
+        es = arr[:, 0]
+        cs = arr[:, 1:] * gto_norm(l, es)[:, None]
+        cs = _normalize_contracted(l, es, cs)
+        nprim, nctr = cs.shape
+        env.append(es)
+        env.append(cs.T.reshape(-1))
+        ptr_exp = ptr
+        ptr_coeff = ptr_exp + nprim
+        ptr = ptr_coeff + nprim * nctr
+        bas.append([atom_id, l, nprim, nctr, 0, ptr_exp, ptr_coeff, 0])
+    bas = numpy.array(bas, dtype=numpy.int32).reshape(-1, BAS_SLOTS)
+    env = numpy.hstack(env) if env else numpy.zeros(0)
+    return bas, env
+
+
+def _parse_atom(atom, unit):
+    if isinstance(atom, str):
+        items = []
+        for line in atom.replace(';', '\n').splitlines():
+            tok = line.split()
+            if not tok:
+                continue
+            if len(tok) != 4:
+                raise ValueError('cannot parse atom line %r' % line)
+            items.append((tok[0], tuple(float(x) for x in tok[1:])))
+    else:
+        items = [(a[0], tuple(float(x) for x in a[1])) for a in atom]
+    if isinstance(unit, str):
+        if unit.upper().startswith(('A', 'ANG')):
+            scale = 1. / BOHR
+        elif unit.upper().startswith('B') or unit.upper().startswith('AU'):
+            scale = 1.
+        else:
+            raise ValueError('unknown unit %r' % unit)
+    else:
+        scale = 1. / float(unit)
+    return [(std_symbol(s), numpy.array(c) * scale) for s, c in items]
+
+
+class Mole:
+    """Minimal molecule object with PySCF's integral-library arrays.
+
+    Attributes after ``build()``: ``_atm, _bas, _env, natm, nbas, cart`` and
+    the methods ``nao_nr, ao_loc_nr, atom_coords, atom_charges,
+    energy_nuc``.  Only Cartesian-free spherical AOs are produced by the
+    J/K engine; ``cart=True`` is accepted for s/p-only bases, where the
+    Cartesian and spherical functions coincide.
+    """
+
+    def __init__(self, atom=None, basis='sto-3g', unit='Angstrom', cart=False,
+                 charge=0, spin=0, verbose=0, **kwargs):
+        self.atom = atom
+        self.basis = basis
+        self.unit = unit
+        self.cart = cart
+        self.charge = charge
+        self.spin = spin
+        self.verbose = verbose
+        self.max_memory = 4000
+        self._atm = self._bas = self._env = None
+        self._built = False
+
+    def build(self, *args, **kwargs):
+        for k, v in kwargs.items():
+            if k in ('atom', 'basis', 'unit', 'cart', 'charge', 'spin', 'verbose'):
+                setattr(self, k, v)
+        self._atom = _parse_atom(self.atom, self.unit)
+        symbols = sorted(set(s for s, _ in self._atom), key=lambda s: [a for a, _ in self._atom].index(s))
+        self._basis = {}
+        for s in symbols:
+            b = self.basis[s] if isinstance(self.basis, dict) and s in self.basis else self.basis
+            if isinstance(b, dict):
+                raise KeyError('no basis given for %s' % s)
+            self._basis[s] = basis_mod.load(b, s) if isinstance(b, str) else list(b)
+
+        env = [numpy.zeros(PTR_ENV_START)]
+        atm = []
+        ptr = PTR_ENV_START
+        for s, coord in self._atom:
+            row = numpy.zeros(ATM_SLOTS, dtype=numpy.int32)
+            row[CHARGE_OF] = charge(s)
+            row[PTR_COORD] = ptr
+            row[NUC_MOD_OF] = NUC_POINT
+            row[PTR_ZETA] = ptr + 3
+            atm.append(row)
+            env.append(numpy.hstack((coord, 0.)))
+            ptr += 4
+        basdic = {}
+        for s in symbols:
+            b0, e0 = make_bas_env(self._basis[s], 0, ptr)
+            ptr += len(e0)
+            basdic[s] = b0
+            env.append(e0)
+        bas = []
+        for ia, (s, _) in enumerate(self._atom):
+            b = basdic[s].copy()
+            b[:, ATOM_OF] = ia
+            bas.append(b)
+        self._atm = numpy.asarray(numpy.vstack(atm), dtype=numpy.int32)
+        self._bas = numpy.asarray(numpy.vstack(bas), dtype=numpy.int32)
+        self._env = numpy.asarray(numpy.hstack(env), dtype=numpy.float64)
+        if self.cart and int(self._bas[:, ANG_OF].max()) > 1:
+            raise NotImplementedError('cart=True is supported for s/p-only bases')
+        self._built = True
+        return self
+
+    # -- properties mirroring pyscf.gto.Mole -------------------------------
+    @property
+    def natm(self):
+        return len(self._atm)
+
+    @property
+    def nbas(self):
+        return len(self._bas)
+
+    @property
+    def nelectron(self):
+        return int(self._atm[:, CHARGE_OF].sum()) - self.charge
+
+    @property
+    def nelec(self):
+        ne = self.nelectron
+        na = (ne + self.spin) // 2
+        return na, ne - na
+
+    def nao_nr(self):
+        return int(self.ao_loc_nr()[-1])
+
+    @property
+    def nao(self):
+        return self.nao_nr()
+
+    def ao_loc_nr(self):
+        l = self._bas[:, ANG_OF]
+        dims = (2 * l + 1) * self._bas[:, NCTR_OF]
+        ao_loc = numpy.zeros(self.nbas + 1, dtype=numpy.int32)
+        ao_loc[1:] = numpy.cumsum(dims)
+        return ao_loc
+
+    def atom_coords(self):
+        ptr = self._atm[:, PTR_COORD]
+        return numpy.array([self._env[p:p + 3] for p in ptr])
+
+    def atom_charges(self):
+        return self._atm[:, CHARGE_OF].astype(float)
+
+    def atom_symbol(self, ia):
+        return self._atom[ia][0]
+
+    def energy_nuc(self):
+        """Nuclear repulsion sum_{A<B} Z_A Z_B / R_AB (pyscf/gto/mole.py:1526)."""
+        z = self.atom_charges()
+        r = self.atom_coords()
+        e = 0.
+        for i in range(len(z)):
+            for j in range(i):
+                e += z[i] * z[j] / numpy.linalg.norm(r[i] - r[j])
+        return e
+
+    def bas_angular(self, i):
+        return int(self._bas[i, ANG_OF])
+
+    def bas_nctr(self, i):
+        return int(self._bas[i, NCTR_OF])
+
+    def bas_nprim(self, i):
+        return int(self._bas[i, NPRIM_OF])
+
+    def bas_exp(self, i):
+        p = self._bas[i, PTR_EXP]
+        return self._env[p:p + self._bas[i, NPRIM_OF]].copy()
+
+    def bas_ctr_coeff(self, i):
+        """Normalised contraction coefficients, shape (nprim, nctr)."""
+        np_, nc = self._bas[i, NPRIM_OF], self._bas[i, NCTR_OF]
+        p = self._bas[i, PTR_COEFF]
+        return self._env[p:p + np_ * nc].reshape(nc, np_).T.copy()
+
+    def bas_coord(self, i):
+        ia = self._bas[i, ATOM_OF]
+        p = self._atm[ia, PTR_COORD]
+        return self._env[p:p + 3].copy()
+
+
+def M(**kwargs):
+    """Shortcut ``Mole(**kwargs).build()``."""
+    return Mole(**kwargs).build()

@@ -1,0 +1,102 @@
+"""Loader for the HIP library and small numpy helpers.
+
+``load_library`` mirrors pyscf/lib/misc.py:95-107 (ctypes load of an
+in-tree shared object); there is no fallback: a missing or unloadable
+library raises.  ``fp`` is pyscf/lib/misc.py:1259-1263 and ``hermi_triu``
+pyscf/lib/numpy_helper.py:480-525.
+"""
+
+import ctypes
+import os
+
+import numpy
+
+_LIBDIR = os.path.dirname(os.path.abspath(__file__))
+_loaded = {}
+
+
+class mjk_stats(ctypes.Structure):
+    _fields_ = [('ms_total', ctypes.c_double),
+                ('ms_quartets', ctypes.c_double),
+                ('ms_transform', ctypes.c_double),
+                ('quartets', ctypes.c_longlong),
+                ('prim_quartets', ctypes.c_longlong),
+                ('flops', ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _declare(lib):
+    c_int, c_double, c_void_p, c_size_t = ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_size_t
+    P = ctypes.POINTER
+    lib.mjk_ctx_create.argtypes = [P(c_void_p), c_int, c_size_t]
+    lib.mjk_ctx_destroy.argtypes = [c_void_p]
+    lib.mjk_ctx_destroy.restype = None
+    lib.mjk_last_error.argtypes = [c_void_p]
+    lib.mjk_last_error.restype = ctypes.c_char_p
+    lib.mjk_ctx_hbm_used.argtypes = [c_void_p]
+    lib.mjk_ctx_hbm_used.restype = c_size_t
+    lib.mjk_basis_upload.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_double, P(c_void_p)]
+    lib.mjk_basis_upload_batch.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_double, P(c_void_p)]
+    lib.mjk_basis_destroy.argtypes = [c_void_p]
+    lib.mjk_basis_destroy.restype = None
+    lib.mjk_basis_nao.argtypes = [c_void_p, c_int]
+    lib.mjk_basis_slot_size.argtypes = [c_void_p]
+    lib.mjk_basis_slot_size.restype = ctypes.c_longlong
+    lib.mjk_basis_q_cond.argtypes = [c_void_p, c_void_p, c_int, c_void_p]
+    lib.mjk_get_jk.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
+                               c_void_p, c_void_p, P(mjk_stats)]
+    lib.mjk_get_jk_device.argtypes = lib.mjk_get_jk.argtypes
+    lib.mjk_device_alloc.argtypes = [c_void_p, c_size_t, P(c_void_p)]
+    lib.mjk_device_free.argtypes = [c_void_p, c_void_p]
+    lib.mjk_memcpy_h2d.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
+    lib.mjk_memcpy_d2h.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
+    lib.mjk_synchronize.argtypes = [c_void_p]
+    lib.mjk_rys_roots.argtypes = [c_int, c_double, c_void_p, c_void_p]
+    for name in ('mjk_int1e',):
+        if hasattr(lib, name):
+            getattr(lib, name).argtypes = [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]
+    return lib
+
+
+def library_path(name='libmjk'):
+    return os.path.join(_LIBDIR, name + '.so')
+
+
+def load_library(name='libmjk'):
+    """ctypes handle of an in-tree library; raises if it is not built."""
+    if name in _loaded:
+        return _loaded[name]
+    path = library_path(name)
+    if not os.path.exists(path):
+        raise OSError('%s is not built; run "python -m pyscf_amd.build"' % path)
+    lib = _declare(ctypes.CDLL(path))
+    _loaded[name] = lib
+    return lib
+
+
+def fp(a):
+    """Fingerprint cos(arange(n)) . a.ravel() (pyscf/lib/misc.py:1259-1263)."""
+    a = numpy.asarray(a)
+    return numpy.dot(numpy.cos(numpy.arange(a.size)), a.ravel())
+
+
+finger = fingerprint = fp
+
+
+def hermi_triu(mat, hermi=1, inplace=True):
+    """Fill the upper triangle from the lower one: hermi=1 mirror,
+    hermi=2 anti-mirror (pyscf/lib/numpy_helper.py:480-525)."""
+    if not inplace:
+        mat = mat.copy()
+    n = mat.shape[-1]
+    iu = numpy.triu_indices(n, 1)
+    if hermi == 1:
+        mat[..., iu[0], iu[1]] = mat[..., iu[1], iu[0]].conj()
+    elif hermi == 2:
+        mat[..., iu[0], iu[1]] = -mat[..., iu[1], iu[0]].conj()
+    else:
+        raise ValueError('hermi must be 1 or 2')
+    return mat

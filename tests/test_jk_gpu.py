@@ -1,0 +1,127 @@
+"""GPU J/K engine against the CPU oracle and the reference's golden values.
+
+FP64 tolerance: 1e-10 max-abs on J and K (the target for the FP64 path)."""
+import numpy
+import pytest
+
+from pyscf_amd import gto, lib
+from pyscf_amd.scf import _vhf, hf
+from tests.conftest import H2O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def c1_density(nao=24):
+    c = numpy.random.default_rng(0).standard_normal((nao, 5))
+    return c @ c.T
+
+
+def test_c1_h2o_ccpvdz_hermi1(oracle, h2o_vdz):
+    dm = c1_density()
+    vj, vk = hf.get_jk(h2o_vdz, dm, hermi=1)
+    j0, k0 = oracle.get_jk(h2o_vdz, dm, hermi=1)
+    assert vj.shape == dm.shape and vk.shape == dm.shape
+    assert abs(vj - j0).max() < TOL
+    assert abs(vk - k0).max() < TOL
+
+
+def test_pin2_nonsymmetric_dm(oracle, pins, h2o_vdz):
+    numpy.random.seed(1)
+    dm = numpy.random.random((24, 24))
+    vj, vk = hf.get_jk(h2o_vdz, dm, hermi=0)
+    assert abs(numpy.linalg.norm(vj) - pins['pin2_h2o_ccpvdz_norm_J']['value']) < 1e-9
+    assert abs(numpy.linalg.norm(vk) - pins['pin2_h2o_ccpvdz_norm_K']['value']) < 1e-9
+    j0, k0 = oracle.get_jk(h2o_vdz, dm, hermi=0)
+    assert abs(vj - j0).max() < TOL and abs(vk - k0).max() < TOL
+
+
+def test_pin3_veff_two_densities(pins, h2o_vdz):
+    numpy.random.seed(1)
+    d1 = numpy.random.random((24, 24))
+    d2 = numpy.random.random((24, 24))
+    v = hf.get_veff(h2o_vdz, (d1 + d1.T, d2 + d2.T))
+    assert v.shape == (2, 24, 24)
+    assert abs(numpy.linalg.norm(v) - pins['pin3_h2o_ccpvdz_norm_veff']['value']) < 1e-9
+
+
+def test_pin4_631g(pins, h2o_631g):
+    numpy.random.seed(1)
+    dm = numpy.random.random((13, 13))
+    dm = dm + dm.T
+    vj, vk = hf.get_jk(h2o_631g, dm, hermi=0)
+    assert abs(lib.fp(vk) - pins['pin4_h2o_631g_fp_K_s1il']['value']) < 1e-11
+    assert abs(lib.fp(vj) - pins['pin4_h2o_631g_fp_J_s1kl']['value']) < 1e-11
+
+
+def test_pin5_complex_cart_screened(pins, oracle):
+    mol = gto.M(atom=H2O, basis='631g', cart=True)
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    numpy.random.seed(1)
+    dm = numpy.random.random((2, 3, 13, 13)) - .5 + 0j
+    vj, vk = hf.get_jk(mol, dm, hermi=0, vhfopt=opt)
+    assert vj.shape == dm.shape and vj.dtype == numpy.complex128
+    assert abs(lib.fp(vj - .5 * vk) - pins['pin5_h2o_631g_cart_fp_vhf4']['value']) < 1e-11
+    vj2, vk2 = hf.get_jk(mol, dm[0, 0], hermi=0, vhfopt=opt)
+    assert vj2.ndim == 2 and abs(vj2 - vj[0, 0]).max() < 1e-12
+
+
+def test_flags_and_hermi2(oracle, h2o_vdz):
+    rng = numpy.random.default_rng(3)
+    a = rng.standard_normal((24, 24))
+    vj, vk = hf.get_jk(h2o_vdz, a + a.T, with_k=False)
+    assert vk is None
+    j0, _ = oracle.get_jk(h2o_vdz, a + a.T)
+    assert abs(vj - j0).max() < TOL
+    vj, vk = hf.get_jk(h2o_vdz, a - a.T, hermi=2, with_j=False)
+    assert vj is None
+    _, k0 = oracle.get_jk(h2o_vdz, a - a.T, hermi=2)
+    assert abs(vk - k0).max() < TOL
+
+
+def test_schwarz_bound_matches_oracle(oracle, h2o_vdz):
+    opt = _vhf.VHFOpt(h2o_vdz, direct_scf_tol=1e-13)
+    q0 = oracle.VHFOpt(h2o_vdz, 1e-13).q_cond
+    q = opt.q_cond
+    assert abs(q - q0).max() < 1e-12 * abs(q0).max()
+
+
+def test_screened_cluster(oracle):
+    mol = gto.M(atom='''O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587
+                        O 3.1 0.2 0; H 3.1 -0.557 0.587; H 3.1 0.957 0.587
+                        O 0 2.9 0.3; H 0 2.143 0.887; H 0 3.657 0.887''', basis='def2-svp')
+    nao = mol.nao_nr()
+    c = numpy.random.default_rng(5).standard_normal((nao, 15))
+    dm = c @ c.T / 15
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    vj, vk = hf.get_jk(mol, dm, vhfopt=opt)
+    j0, k0 = oracle.get_jk(mol, dm, vhfopt=oracle.VHFOpt(mol, 1e-13))
+    assert abs(vj - j0).max() < TOL and abs(vk - k0).max() < TOL
+    j1, k1 = hf.get_jk(mol, dm)
+    assert abs(vj - j1).max() < 1e-9 and abs(vk - k1).max() < 1e-9
+
+
+def test_run_to_run_determinism(h2o_vdz):
+    dm = c1_density()
+    outs = [hf.get_jk(h2o_vdz, dm) for _ in range(3)]
+    for vj, vk in outs[1:]:
+        assert abs(vj - outs[0][0]).max() < 1e-13 and abs(vk - outs[0][1]).max() < 1e-13
+
+
+def test_batch_equals_single(oracle):
+    mols = [gto.M(atom=H2O, basis='cc-pvdz'), gto.M(atom='O 0 0 0; H 0 0 0.96; H 0.93 0 -0.25', basis='631g')]
+    eng = _vhf.get_engine()
+    basis = eng.upload_batch(mols, 0.0)
+    dms = []
+    for m in mols:
+        c = numpy.random.default_rng(m.nao_nr()).standard_normal((m.nao_nr(), 4))
+        dms.append(c @ c.T)
+    flat = numpy.concatenate([d.ravel() for d in dms])[None]
+    vj, vk = basis.get_jk(flat, 1, True, True, 0.0)
+    off = 0
+    for m, d in zip(mols, dms):
+        n = m.nao_nr()
+        j0, k0 = oracle.get_jk(m, d)
+        assert abs(vj[0, off:off + n * n].reshape(n, n) - j0).max() < TOL
+        assert abs(vk[0, off:off + n * n].reshape(n, n) - k0).max() < TOL
+        off += n * n
