@@ -1,0 +1,191 @@
+"""Benchmark of the direct J/K build (get_jk) on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload c2] [--no-cpu]
+
+Workload (default "c2"): (H2O)16 on a jittered 3.0 Angstrom cubic lattice,
+def2-SVP (384 AOs), RHF converged to 1e-10 Eh with the GPU engine; then one
+step = one J/K build (J and K of the converged density, hermi=1,
+direct_scf_tol=1e-13) with the density already resident in HBM.  Every
+step is synchronised so that its latency is a sample of the p50/p99.
+
+Printed: one JSON line with value = JK builds/s (whole job, one GPU),
+quartets/s, p50/p99 latency, the roofline of the dominant quartet-class
+kernel (algorithmic FP64 FLOPs / its HIP-event duration on the launch
+stream, against 78.6 TFLOP/s) and the CPU baseline (the oracle's OpenMP
+restatement of the reference's direct J/K loop, timed on a bounded sample of
+bra shell pairs of the same build and scaled to builds/s).
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'JK builds/s + ERI quartets/s, mixed vs FP64, at <=1e-8 Eh; p50/p99 latency'
+FP64_PEAK = 78.6e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='c2', choices=['c1', 'c2', 'c3'])
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--cpu-seconds', type=float, default=10.0)
+    return ap.parse_args()
+
+
+def workload(name):
+    from pyscf_amd import gto, workloads
+    if name == 'c1':
+        return gto.M(atom=workloads.h2o(), basis='cc-pvdz'), 'H2O cc-pVDZ (24 AOs)'
+    if name == 'c2':
+        return (gto.M(atom=workloads.water_cluster(16, seed=1), basis='def2-svp'),
+                '(H2O)16 jittered 3.0 A lattice (seed 1), def2-SVP (384 AOs)')
+    return gto.M(atom=workloads.alkane(30), basis='cc-pvdz'), 'C30H62 all-trans, cc-pVDZ (730 AOs)'
+
+
+def cpu_baseline(mol, dm, tol, seconds):
+    """Oracle (OpenMP C) on a bounded sample of bra shell pairs."""
+    from oracle import oracle as O
+    ncores = len(os.sched_getaffinity(0))
+    O.set_threads(ncores)
+    opt = O.VHFOpt(mol, tol)
+    dms = dm[None]
+    _, _, n_total = O.direct_full(mol, dms, opt, pair_stride=0)
+    t = time.perf_counter()
+    _, _, n_probe = O.direct_full(mol, dms, opt, pair_stride=97)
+    rate = n_probe / max(time.perf_counter() - t, 1e-6)
+    stride = max(1, int(n_total / max(rate * seconds, 1)))
+    t = time.perf_counter()
+    _, _, n_s = O.direct_full(mol, dms, opt, pair_stride=stride)
+    dt = time.perf_counter() - t
+    qps = n_s / dt
+    return {'value': qps / n_total, 'unit': 'JK builds/s', 'cores': ncores, 'kind': 'port',
+            'quartets_per_s': qps,
+            'sample': 'bra shell pairs ij with ij %% %d == 0 of the same screened build: %d of %d '
+                      'unique shell quartets in %.2f s, scaled by quartet count' % (stride, n_s, n_total, dt)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md); '
+              'got --gpus %d' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    from pyscf_amd import lib
+    from pyscf_amd.scf import _vhf, rhf
+
+    mol, desc = workload(args.workload)
+    nao = mol.nao_nr()
+    tol = 1e-13
+    t0 = time.perf_counter()
+    mf = rhf.RHF(mol, direct_scf_tol=tol)
+    mf.conv_tol = 1e-10
+    e64 = mf.kernel()
+    t_scf = time.perf_counter() - t0
+    dm = numpy.ascontiguousarray(mf.dm)
+
+    eng = mf._opt.engine
+    L = eng.lib
+    basis = mf._opt.basis
+    ctx = eng.handle
+    nbytes = dm.nbytes
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    d_dm, d_vj, d_vk = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (d_dm, d_vj, d_vk):
+        eng.check(L.mjk_device_alloc(ctx, nbytes, ctypes.byref(p)), 'device_alloc')
+    eng.check(L.mjk_memcpy_h2d(ctx, d_dm, ptr(dm), nbytes), 'h2d')
+
+    def build(stats=None):
+        rc = L.mjk_get_jk_device(ctx, basis.handle, d_dm, 1, 1, 1, 1, tol, 0.0, d_vj, d_vk,
+                                 ctypes.byref(stats) if stats is not None else None)
+        eng.check(rc, 'mjk_get_jk_device')
+
+    st = lib.mjk_stats()
+    build(st)                      # first call: tile lists for this cutoff, work buffers
+    for _ in range(args.warmup):
+        build()
+    eng.check(L.mjk_synchronize(ctx), 'sync')
+    # timed region: per-class HIP events on the launch stream, every step synchronised
+    eng.check(L.mjk_profile(ctx, basis.handle, 1), 'profile')
+    lat = []
+    eng.check(L.mjk_synchronize(ctx), 'sync')
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        build()
+        eng.check(L.mjk_synchronize(ctx), 'sync')
+        lat.append(time.perf_counter() - t)
+    eng.check(L.mjk_synchronize(ctx), 'sync')
+    t_total = time.perf_counter() - t_start
+    maxc = 64
+    cls = numpy.zeros(4 * maxc, dtype=numpy.int32)
+    ms = numpy.zeros(maxc)
+    nq = numpy.zeros(maxc, dtype=numpy.int64)
+    npq = numpy.zeros(maxc, dtype=numpy.int64)
+    fl = numpy.zeros(maxc)
+    nl = numpy.zeros(maxc, dtype=numpy.int64)
+    k = L.mjk_profile_read(ctx, basis.handle, maxc, ptr(cls), ptr(ms), ptr(nq), ptr(npq), ptr(fl), ptr(nl))
+    L.mjk_profile(ctx, basis.handle, 0)
+    classes = []
+    for i in range(k):
+        classes.append({'class': '(%d%d|%d%d)' % tuple(cls[4 * i:4 * i + 4]), 'ms_per_launch': ms[i] / nl[i],
+                        'quartets': int(nq[i] // nl[i]), 'flops': fl[i] / nl[i],
+                        'tflops': fl[i] / (ms[i] * 1e-3) / 1e12 if ms[i] > 0 else 0.0})
+    classes.sort(key=lambda c: -c['ms_per_launch'])
+    dom = classes[0]
+    vj = numpy.empty_like(dm)
+    vk = numpy.empty_like(dm)
+    eng.check(L.mjk_memcpy_d2h(ctx, ptr(vj), d_vj, nbytes), 'd2h')
+    eng.check(L.mjk_memcpy_d2h(ctx, ptr(vk), d_vk, nbytes), 'd2h')
+    for p in (d_dm, d_vj, d_vk):
+        L.mjk_device_free(ctx, p)
+
+    ms_step = t_total / args.steps * 1e3
+    builds = args.steps / t_total
+    kernel_ms = sum(c['ms_per_launch'] for c in classes)
+    total_flops = sum(c['flops'] for c in classes)
+    out = {
+        'metric': METRIC,
+        'value': builds,
+        'unit': 'JK builds/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic geometry; converged RHF density from this engine',
+        'config': {'workload': desc + ', one J/K build of the converged RHF density, hermi=1, '
+                   'direct_scf_tol=1e-13', 'nao': nao, 'n_dm': 1, 'precision': 'fp64'},
+        'quartets_per_s': st.quartets * builds,
+        'quartets_per_build': int(st.quartets),
+        'p50_ms': float(numpy.percentile(lat, 50) * 1e3),
+        'p99_ms': float(numpy.percentile(lat, 99) * 1e3),
+        'scf': {'e_tot': e64, 'cycles': mf.cycles, 'converged': mf.converged, 'seconds': t_scf},
+        'gpu_kernel_ms_per_build': kernel_ms,
+        'fp64_tflops_all_classes': total_flops / (kernel_ms * 1e-3) / 1e12,
+        'roofline': {'bound': 'fp64', 'kernel': 'k_jk_lane' + dom['class'],
+                     'achieved': dom['tflops'], 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
+                     'frac': dom['tflops'] * 1e12 / FP64_PEAK, 'traffic': None,
+                     'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch']},
+        'classes': classes[:8],
+    }
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline(mol, dm, tol, args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

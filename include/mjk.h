@@ -111,12 +111,31 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n
                       int with_j, int with_k, double direct_scf_tol, double fp32_tol,
                       double* d_vj, double* d_vk, mjk_stats* stats);
 
+/* Per-quartet-class kernel profiling with HIP events on the launch stream.
+ * mjk_profile(…, 1) resets and starts accumulating (each J/K build then
+ * synchronises at its end); mjk_profile_read returns the number of classes
+ * written: cls[4k..4k+3] = (la lb lc ld), accumulated GPU ms, evaluated
+ * quartets, primitive quartets, algorithmic FLOPs and launches.  No
+ * reference counterpart (the reference's timer is logger.timer,
+ * pyscf/lib/logger.py:167-180). */
+int mjk_profile(mjk_ctx* ctx, mjk_basis* basis, int enable);
+int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* basis, int max_classes, int* cls, double* ms,
+                     long long* nq, long long* nprimq, double* flops, long long* launches);
+
 /* Device memory helpers (for callers without another GPU runtime). */
 int mjk_device_alloc(mjk_ctx* ctx, size_t bytes, void** ptr);
 int mjk_device_free(mjk_ctx* ctx, void* ptr);
 int mjk_memcpy_h2d(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
 int mjk_memcpy_d2h(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
 int mjk_synchronize(mjk_ctx* ctx);
+
+/* One-electron integrals for the SCF caller, computed on the host:
+ * which = 0 overlap (int1e_ovlp), 1 kinetic (int1e_kin), 2 nuclear
+ * attraction (int1e_nuc); out float64 [nao][nao] spherical.  Replaces
+ * mol.intor('int1e_ovlp'/'int1e_kin'/'int1e_nuc') as used by
+ * get_ovlp/get_hcore (pyscf/scf/hf.py:315-345, pyscf/gto/mole.py:3474). */
+int mjk_int1e(int which, const int* atm, int natm, const int* bas, int nbas, const double* env,
+              double* out);
 
 /* Rys roots x_i (= t_i^2) and weights of the interpolation table, evaluated
  * on the host (table validation).  n <= 9.  No reference counterpart (the

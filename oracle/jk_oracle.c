@@ -633,7 +633,8 @@ static int prescreen(int i, int j, int k, int l, const double *q, const double *
  * the reference's lower-triangle/mirror convention.  q_cond == NULL means no
  * screening (module-level get_jk without an optimizer).  If pair_stride > 1
  * only bra pairs ij with ij % pair_stride == pair_offset are processed (a
- * bounded sample for timing).  Returns the number of evaluated quartets. */
+ * bounded sample for timing).  pair_stride == 0 only counts the quartets
+ * that pass the screen (no ERI).  Returns the number of evaluated quartets. */
 long orc_direct_jk(const double *dms, int n_dm, double *vj, double *vk,
                    int with_j, int with_k,
                    const double *q_cond, const double *dm_cond, double cutoff,
@@ -651,6 +652,7 @@ long orc_direct_jk(const double *dms, int n_dm, double *vj, double *vk,
         if (with_k) memset(vk, 0, sizeof(double) * nn * n_dm);
         size_t npair = (size_t)nbas * (nbas + 1) / 2;
         long nquartets = 0;
+        const int count_only = (pair_stride == 0);
         if (pair_stride < 1) pair_stride = 1;
 #pragma omp parallel reduction(+:nquartets)
 {
@@ -670,8 +672,9 @@ long orc_direct_jk(const double *dms, int n_dm, double *vj, double *vk,
                         if (q_cond && !prescreen(ish, jsh, ksh, lsh, q_cond, dm_cond, nbas, cutoff))
                                 continue;
                         int shls[4] = {ish, jsh, ksh, lsh};
-                        orc_int2e_sph(buf, shls, atm, natm, bas, nbas, env);
                         nquartets++;
+                        if (count_only) continue;
+                        orc_int2e_sph(buf, shls, atm, natm, bas, nbas, env);
                         double f = 1.;
                         if (ish == jsh) f *= .5;
                         if (ksh == lsh) f *= .5;

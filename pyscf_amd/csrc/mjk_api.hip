@@ -42,6 +42,7 @@ int launch_cart2sph(const XformArgs& x, int maxnao, const double* Jc, const doub
                     hipStream_t st);
 int launch_dmcond(const DmCondArgs& x, const double* Dsph, long long sslot, int n_dm, double* out,
                   hipStream_t st);
+int lmax_kernels();
 }  // namespace mjk
 
 using namespace mjk;
@@ -78,7 +79,7 @@ struct QClass {
 
 int lmax_built()
 {
-    return 2;   // quartet kernels instantiated for l <= 2
+    return lmax_kernels();
 }
 
 double flops_class(int a, int b, int c, int d, double nq, double nprimq)
@@ -198,6 +199,11 @@ struct mjk_basis {
     long long cap_ds = 0, cap_vj = 0, cap_vk = 0, cap_dj = 0, cap_j = 0, cap_dk = 0, cap_k = 0, cap_cond = 0;
     unsigned long long* counters = nullptr;
     std::vector<void*> owned;
+    // per-class profiling (HIP events around every class launch)
+    bool profiling = false;
+    std::vector<hipEvent_t> pev;          // 2 per quartet class
+    std::vector<double> prof_ms;
+    std::vector<long long> prof_nq, prof_np, prof_launches;
 };
 
 namespace {
@@ -730,6 +736,7 @@ void mjk_basis_destroy(mjk_basis* b)
     for (void* p : b->owned) dfree(ctx, p);
     double* work[] = {b->Dsph, b->Vj, b->Vk, b->Dj, b->Dk, b->J, b->K, b->dmcond};
     for (double* p : work) dfree(ctx, p);
+    for (hipEvent_t e : b->pev) (void)hipEventDestroy(e);
     delete b;
 }
 
@@ -814,9 +821,11 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.cutoff = cut;
     a.screen = screen ? 1 : 0;
     a.rys = ctx->rys;
+    const bool prof = b->profiling && b->pev.size() >= 2 * b->qcls.size();
     for (size_t c = 0; c < b->qcls.size(); ++c) {
         const QClass& qc = b->qcls[c];
         if (qc.nblocks == 0) continue;
+        if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c], st));
         a.bt = qc.bt;
         a.bt_mol = qc.bt_mol;
         a.bt_off = qc.bt_off;
@@ -825,12 +834,28 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         a.counters = b->counters + 2 * c;
         if (launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
             return fail(ctx, MJK_EUNSUPPORTED, "no kernel for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
+        if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c + 1], st));
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
     if (launch_cart2sph(xform(b), b->maxnao, b->J, b->K, b->slot_c, n_dm, hermi, with_j, with_k, d_vj, d_vk,
                         b->slot_s, st))
         return fail(ctx, MJK_EDEVICE, "output transform launch failed");
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
+    if (prof) {
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        std::vector<unsigned long long> cnt(2 * NQCLASS_MAX);
+        HIPCHK(ctx, hipMemcpy(cnt.data(), b->counters, sizeof(unsigned long long) * cnt.size(),
+                              hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < b->qcls.size(); ++c) {
+            if (b->qcls[c].nblocks == 0) continue;
+            float ms = 0;
+            HIPCHK(ctx, hipEventElapsedTime(&ms, b->pev[2 * c], b->pev[2 * c + 1]));
+            b->prof_ms[c] += ms;
+            b->prof_nq[c] += (long long)cnt[2 * c];
+            b->prof_np[c] += (long long)cnt[2 * c + 1];
+            b->prof_launches[c] += 1;
+        }
+    }
     if (stats) {
         HIPCHK(ctx, hipStreamSynchronize(st));
         float t01 = 0, t12 = 0, t23 = 0;
@@ -883,6 +908,47 @@ int mjk_get_jk(mjk_ctx* ctx, mjk_basis* b, const double* dms, int n_dm, int herm
         stats->ms_total =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return MJK_OK;
+}
+
+int mjk_profile(mjk_ctx* ctx, mjk_basis* b, int enable)
+{
+    if (!ctx || !b) return MJK_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t n = NQCLASS_MAX;
+    if (b->pev.empty()) {
+        b->pev.resize(2 * n);
+        for (auto& e : b->pev) HIPCHK(ctx, hipEventCreate(&e));
+    }
+    b->prof_ms.assign(n, 0.0);
+    b->prof_nq.assign(n, 0);
+    b->prof_np.assign(n, 0);
+    b->prof_launches.assign(n, 0);
+    b->profiling = enable != 0;
+    return MJK_OK;
+}
+
+int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* b, int max_classes, int* cls, double* ms, long long* nq,
+                     long long* nprimq, double* flops, long long* launches)
+{
+    if (!ctx || !b) return MJK_EINVAL;
+    int k = 0;
+    for (size_t c = 0; c < b->qcls.size() && c < b->prof_ms.size(); ++c) {
+        if (b->prof_launches[c] == 0) continue;
+        if (k >= max_classes) break;
+        const QClass& qc = b->qcls[c];
+        cls[4 * k] = qc.la;
+        cls[4 * k + 1] = qc.lb;
+        cls[4 * k + 2] = qc.lc;
+        cls[4 * k + 3] = qc.ld;
+        ms[k] = b->prof_ms[c];
+        nq[k] = b->prof_nq[c];
+        nprimq[k] = b->prof_np[c];
+        flops[k] = flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)b->prof_nq[c], (double)b->prof_np[c]);
+        launches[k] = b->prof_launches[c];
+        ++k;
+    }
+    return k;
 }
 
 int mjk_device_alloc(mjk_ctx* ctx, size_t bytes, void** ptr)
@@ -950,3 +1016,179 @@ int mjk_rys_roots(int n, double T, double* x, double* w)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// One-electron integrals on the host for the SCF caller (overlap, kinetic,
+// nuclear attraction), spherical AOs in PySCF order.  Obara-Saika 1D overlap
+// recurrences; nuclear attraction by Rys quadrature with the same root
+// table as the ERI kernels: V = -Z 2pi/p K_ab sum_i w_i Ix Iy Iz with
+// C00 = (P-A) - t^2 (P-C), B10 = (1-t^2)/(2p).
+namespace {
+
+void overlap_1d(double S[8][8], int la, int lb, double p, double PA, double PB, double s00)
+{
+    const double h = 0.5 / p;
+    S[0][0] = s00;
+    for (int a = 0; a <= la; ++a)
+        for (int b = 0; b <= lb; ++b) {
+            if (a == 0 && b == 0) continue;
+            double v;
+            if (a > 0) {
+                v = PA * S[a - 1][b];
+                if (a > 1) v += h * (a - 1) * S[a - 2][b];
+                if (b > 0) v += h * b * S[a - 1][b - 1];
+            } else {
+                v = PB * S[a][b - 1];
+                if (b > 1) v += h * (b - 1) * S[a][b - 2];
+            }
+            S[a][b] = v;
+        }
+}
+
+template <int N>
+void nuc_roots(const double* tab, double T, double* x, double* w)
+{
+    double xx[N], ww[N];
+    rys_roots<N>(tab, T, xx, ww);
+    for (int i = 0; i < N; ++i) {
+        x[i] = xx[i];
+        w[i] = ww[i];
+    }
+}
+
+void roots_n(const double* tab, int n, double T, double* x, double* w)
+{
+    switch (n) {
+    case 1: nuc_roots<1>(tab, T, x, w); break;
+    case 2: nuc_roots<2>(tab, T, x, w); break;
+    case 3: nuc_roots<3>(tab, T, x, w); break;
+    case 4: nuc_roots<4>(tab, T, x, w); break;
+    default: nuc_roots<5>(tab, T, x, w); break;
+    }
+}
+
+int cart_xyz(int l, int i, int& lx, int& ly)
+{
+    lx = l;
+    int k = i;
+    while (k > l - lx) { k -= l - lx + 1; --lx; }
+    ly = l - lx - k;
+    return l - lx - ly;
+}
+
+}  // namespace
+
+extern "C" int mjk_int1e(int which, const int* atm, int natm, const int* bas, int nbas, const double* env,
+                         double* out)
+{
+    if (which < 0 || which > 2 || !out) return MJK_EINVAL;
+    MolHost m;
+    if (parse_molecule(nullptr, atm, natm, bas, nbas, env, m) != MJK_OK) return MJK_EINVAL;
+    static std::vector<double> tab;
+    static std::mutex tab_mu;
+    {
+        std::lock_guard<std::mutex> lk(tab_mu);
+        if (tab.empty()) {
+            tab.resize(RYS_BUF_SIZE);
+            rys_fill_buffer(tab.data());
+        }
+    }
+    const int nao = m.nao;
+    for (int i = 0; i < m.nsh; ++i)
+        for (int j = 0; j < m.nsh; ++j) {
+            const GShell& A = m.sh[i];
+            const GShell& B = m.sh[j];
+            const int la = A.l, lb = B.l, na = ncart(la), nb = ncart(lb);
+            double cart[10][10];
+            for (auto& r : cart) for (double& v : r) v = 0.0;
+            double AB2 = 0;
+            for (int d = 0; d < 3; ++d) AB2 += (A.r[d] - B.r[d]) * (A.r[d] - B.r[d]);
+            for (int ia = 0; ia < A.nprim; ++ia)
+                for (int ib = 0; ib < B.nprim; ++ib) {
+                    const double a = A.e[ia], b = B.e[ib], p = a + b;
+                    const double cc = A.c[ia] * B.c[ib];
+                    const double K = std::exp(-a * b / p * AB2);
+                    double P[3];
+                    for (int d = 0; d < 3; ++d) P[d] = (a * A.r[d] + b * B.r[d]) / p;
+                    if (which <= 1) {
+                        double S[3][8][8];
+                        for (int d = 0; d < 3; ++d)
+                            overlap_1d(S[d], la + 1, lb + 2, p, P[d] - A.r[d], P[d] - B.r[d],
+                                       std::sqrt(M_PI / p) * (d == 0 ? K : 1.0));
+                        for (int x = 0; x < na; ++x)
+                            for (int y = 0; y < nb; ++y) {
+                                int e1[3], e2[3];
+                                e1[2] = cart_xyz(la, x, e1[0], e1[1]);
+                                e2[2] = cart_xyz(lb, y, e2[0], e2[1]);
+                                double v;
+                                if (which == 0) {
+                                    v = S[0][e1[0]][e2[0]] * S[1][e1[1]][e2[1]] * S[2][e1[2]][e2[2]];
+                                } else {
+                                    v = 0.0;
+                                    for (int d = 0; d < 3; ++d) {
+                                        const int u = e1[d], t = e2[d];
+                                        double kd = -2.0 * b * (2 * t + 1) * S[d][u][t] + 4.0 * b * b * S[d][u][t + 2];
+                                        if (t >= 2) kd += t * (t - 1) * S[d][u][t - 2];
+                                        kd *= -0.5;
+                                        double o = 1.0;
+                                        for (int d2 = 0; d2 < 3; ++d2)
+                                            if (d2 != d) o *= S[d2][e1[d2]][e2[d2]];
+                                        v += kd * o;
+                                    }
+                                }
+                                cart[x][y] += cc * v;
+                            }
+                    } else {
+                        const int nr = (la + lb) / 2 + 1;
+                        for (int c = 0; c < natm; ++c) {
+                            const double* C = env + atm[c * ATM_SLOTS + PTR_COORD];
+                            const double Z = atm[c * ATM_SLOTS];
+                            double PC[3], PC2 = 0;
+                            for (int d = 0; d < 3; ++d) {
+                                PC[d] = P[d] - C[d];
+                                PC2 += PC[d] * PC[d];
+                            }
+                            double x[9], w[9];
+                            roots_n(tab.data(), nr, p * PC2, x, w);
+                            const double fac = -Z * 2.0 * M_PI / p * K * cc;
+                            for (int r = 0; r < nr; ++r) {
+                                double I[3][4][4];
+                                for (int d = 0; d < 3; ++d) {
+                                    double g[8];
+                                    const double C00 = P[d] - A.r[d] - x[r] * PC[d];
+                                    const double B10 = 0.5 / p * (1.0 - x[r]);
+                                    g[0] = (d == 2) ? w[r] * fac : 1.0;
+                                    if (la + lb > 0) g[1] = C00 * g[0];
+                                    for (int k = 1; k < la + lb; ++k) g[k + 1] = C00 * g[k] + k * B10 * g[k - 1];
+                                    double h[8][4];
+                                    for (int k = 0; k <= la + lb; ++k) h[k][0] = g[k];
+                                    for (int t = 1; t <= lb; ++t)
+                                        for (int k = 0; k <= la + lb - t; ++k)
+                                            h[k][t] = h[k + 1][t - 1] + (A.r[d] - B.r[d]) * h[k][t - 1];
+                                    for (int u = 0; u <= la; ++u)
+                                        for (int t = 0; t <= lb; ++t) I[d][u][t] = h[u][t];
+                                }
+                                for (int xx = 0; xx < na; ++xx)
+                                    for (int y = 0; y < nb; ++y) {
+                                        int e1[3], e2[3];
+                                        e1[2] = cart_xyz(la, xx, e1[0], e1[1]);
+                                        e2[2] = cart_xyz(lb, y, e2[0], e2[1]);
+                                        cart[xx][y] += I[0][e1[0]][e2[0]] * I[1][e1[1]][e2[1]] * I[2][e1[2]][e2[2]];
+                                    }
+                            }
+                        }
+                    }
+                }
+            for (int mi = 0; mi < nsph(la); ++mi)
+                for (int mj = 0; mj < nsph(lb); ++mj) {
+                    double v = 0.0;
+                    for (int x = 0; x < na; ++x) {
+                        const double c1 = C2S[la][mi][x];
+                        if (c1 == 0.0) continue;
+                        for (int y = 0; y < nb; ++y) v += c1 * C2S[lb][mj][y] * cart[x][y];
+                    }
+                    out[(size_t)(A.s0 + mi) * nao + B.s0 + mj] = v;
+                }
+        }
+    return MJK_OK;
+}

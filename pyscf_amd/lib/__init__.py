@@ -55,6 +55,8 @@ def _declare(lib):
     lib.mjk_memcpy_d2h.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
     lib.mjk_synchronize.argtypes = [c_void_p]
     lib.mjk_rys_roots.argtypes = [c_int, c_double, c_void_p, c_void_p]
+    lib.mjk_profile.argtypes = [c_void_p, c_void_p, c_int]
+    lib.mjk_profile_read.argtypes = [c_void_p, c_void_p, c_int] + [c_void_p] * 6
     for name in ('mjk_int1e',):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]

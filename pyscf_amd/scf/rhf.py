@@ -1,0 +1,187 @@
+"""Closed-shell RHF driver that calls the GPU J/K engine unchanged.
+
+Restates the reference SCF loop (pyscf/scf/hf.py:47-242):
+  * fock = h + vhf, CDIIS extrapolation on the commutator S D F - F D S
+    (pyscf/scf/diis.py:40-66, pyscf/lib/diis.py:202-275), from cycle 1;
+  * generalized eigenproblem F C = S C e (pyscf/scf/hf.py:1340-1347);
+  * aufbau occupation, D = 2 C_occ C_occ^T (pyscf/scf/hf.py:840);
+  * incremental Fock build vhf = vhf_last + J(dD) - K(dD)/2 with screening
+    at direct_scf_tol (pyscf/scf/hf.py:2416-2427);
+  * converged when |dE| < conv_tol and |g| < sqrt(conv_tol)
+    (pyscf/scf/hf.py:194-197).
+The initial guess is the core Hamiltonian ('1e'); the reference defaults to
+'minao', which changes the iteration path but not the converged energy.
+One-electron integrals come from the library's host code (mjk_int1e).
+"""
+
+import ctypes
+import time
+
+import numpy
+import scipy.linalg
+
+from pyscf_amd import lib as _lib
+from pyscf_amd.scf import _vhf, hf
+
+
+def int1e(mol, which):
+    """'ovlp', 'kin' or 'nuc' matrix (spherical AOs)."""
+    l = _lib.load_library()
+    code = {'ovlp': 0, 'kin': 1, 'nuc': 2}[which]
+    atm = numpy.ascontiguousarray(mol._atm, dtype=numpy.int32)
+    bas = numpy.ascontiguousarray(mol._bas, dtype=numpy.int32)
+    env = numpy.ascontiguousarray(mol._env, dtype=numpy.double)
+    nao = int(mol.nao_nr())
+    out = numpy.zeros((nao, nao))
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rc = l.mjk_int1e(code, p(atm), len(atm), p(bas), len(bas), p(env), p(out))
+    if rc != 0:
+        raise RuntimeError('mjk_int1e failed (%d)' % rc)
+    return out
+
+
+class CDIIS:
+    """Commutator DIIS on the Fock matrix."""
+
+    def __init__(self, space=8):
+        self.space = space
+        self.fs = []
+        self.es = []
+
+    def update(self, s, d, f):
+        sdf = s @ d @ f
+        self.fs.append(f)
+        self.es.append(sdf.T - sdf)
+        self.fs = self.fs[-self.space:]
+        self.es = self.es[-self.space:]
+        n = len(self.fs)
+        if n < 2:
+            return f
+        B = numpy.empty((n + 1, n + 1))
+        B[0, 0] = 0
+        B[0, 1:] = B[1:, 0] = 1
+        for i in range(n):
+            for j in range(i + 1):
+                B[i + 1, j + 1] = B[j + 1, i + 1] = numpy.vdot(self.es[i], self.es[j])
+        rhs = numpy.zeros(n + 1)
+        rhs[0] = 1
+        try:
+            c = scipy.linalg.solve(B, rhs, assume_a='sym')[1:]
+        except (numpy.linalg.LinAlgError, ValueError):
+            c = numpy.linalg.lstsq(B, rhs, rcond=None)[0][1:]
+        return sum(ci * fi for ci, fi in zip(c, self.fs))
+
+
+class RHF:
+    def __init__(self, mol, direct_scf_tol=1e-13, fp32_tol=0.0):
+        self.mol = mol
+        self.conv_tol = 1e-9
+        self.conv_tol_grad = None
+        self.max_cycle = 100
+        self.direct_scf_tol = direct_scf_tol
+        self.fp32_tol = fp32_tol
+        self.diis_space = 8
+        self.verbose = 0
+        self._opt = None
+        self.e_tot = 0.0
+        self.converged = False
+        self.cycles = 0
+        self.jk_times = []
+        self.mo_energy = self.mo_coeff = self.mo_occ = None
+
+    # -- integrals -------------------------------------------------------
+    def get_hcore(self, mol=None):
+        mol = mol or self.mol
+        return int1e(mol, 'kin') + int1e(mol, 'nuc')
+
+    def get_ovlp(self, mol=None):
+        return int1e(mol or self.mol, 'ovlp')
+
+    def init_direct_scf(self, mol=None):
+        return _vhf.VHFOpt(mol or self.mol, direct_scf_tol=self.direct_scf_tol)
+
+    def get_jk(self, mol=None, dm=None, hermi=1, with_j=True, with_k=True, omega=None):
+        mol = mol or self.mol
+        if self._opt is None:
+            self._opt = self.init_direct_scf(mol)
+        t0 = time.perf_counter()
+        vj, vk = hf.get_jk(mol, dm, hermi, self._opt, with_j, with_k, omega, fp32_tol=self.fp32_tol)
+        self.jk_times.append(time.perf_counter() - t0)
+        return vj, vk
+
+    def get_veff(self, mol=None, dm=None, dm_last=None, vhf_last=None, hermi=1):
+        if dm_last is None or vhf_last is None:
+            vj, vk = self.get_jk(mol, dm, hermi)
+            return vj - vk * .5
+        vj, vk = self.get_jk(mol, numpy.asarray(dm) - numpy.asarray(dm_last), hermi)
+        return vj - vk * .5 + vhf_last
+
+    # -- SCF pieces --------------------------------------------------------
+    @staticmethod
+    def eig(f, s):
+        return scipy.linalg.eigh(f, s)
+
+    def get_occ(self, mo_energy):
+        nocc = self.mol.nelectron // 2
+        occ = numpy.zeros_like(mo_energy)
+        occ[numpy.argsort(mo_energy)[:nocc]] = 2
+        return occ
+
+    @staticmethod
+    def make_rdm1(mo_coeff, mo_occ):
+        c = mo_coeff[:, mo_occ > 0]
+        return (c * mo_occ[mo_occ > 0]) @ c.T
+
+    @staticmethod
+    def energy_elec(dm, h1e, vhf):
+        e1 = numpy.einsum('ij,ji->', h1e, dm)
+        e2 = .5 * numpy.einsum('ij,ji->', vhf, dm)
+        return e1 + e2
+
+    def get_grad(self, mo_coeff, mo_occ, fock):
+        occ = mo_occ > 0
+        return (mo_coeff[:, ~occ].T @ fock @ mo_coeff[:, occ]) * 2
+
+    def kernel(self, dm0=None):
+        mol = self.mol
+        if mol.nelectron % 2:
+            raise ValueError('RHF needs an even number of electrons')
+        h1e = self.get_hcore()
+        s1e = self.get_ovlp()
+        enuc = mol.energy_nuc()
+        conv_tol_grad = self.conv_tol_grad or numpy.sqrt(self.conv_tol)
+        if dm0 is None:
+            e, c = self.eig(h1e, s1e)
+            dm = self.make_rdm1(c, self.get_occ(e))
+        else:
+            dm = dm0
+        vhf = self.get_veff(mol, dm)
+        e_tot = self.energy_elec(dm, h1e, vhf) + enuc
+        diis = CDIIS(self.diis_space)
+        self.converged = False
+        for cycle in range(self.max_cycle):
+            dm_last, last_e = dm, e_tot
+            fock = h1e + vhf
+            fock = diis.update(s1e, dm, fock)
+            mo_energy, mo_coeff = self.eig(fock, s1e)
+            mo_occ = self.get_occ(mo_energy)
+            dm = self.make_rdm1(mo_coeff, mo_occ)
+            vhf = self.get_veff(mol, dm, dm_last, vhf)
+            e_tot = self.energy_elec(dm, h1e, vhf) + enuc
+            fock = h1e + vhf
+            norm_g = numpy.linalg.norm(self.get_grad(mo_coeff, mo_occ, fock))
+            if self.verbose:
+                print('cycle %d E=%.15g dE=%.3g |g|=%.3g' % (cycle + 1, e_tot, e_tot - last_e, norm_g))
+            self.cycles = cycle + 1
+            if abs(e_tot - last_e) < self.conv_tol and norm_g < conv_tol_grad:
+                self.converged = True
+                break
+        self.e_tot = e_tot
+        self.mo_energy, self.mo_coeff, self.mo_occ = mo_energy, mo_coeff, mo_occ
+        self.dm = dm
+        return e_tot
+
+    scf = kernel
+
+    def make_rdm1_final(self):
+        return self.make_rdm1(self.mo_coeff, self.mo_occ)
