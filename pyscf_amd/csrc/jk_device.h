@@ -151,58 +151,75 @@ __device__ __forceinline__ void count_work(unsigned long long* c, bool alive, in
 }
 
 // ---------------------------------------------------------------------------
-// One lane per shell quartet.
-template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(64) void k_jk_lane(const JKArgs a)
+// Quartet kernel.  A block owns one 8 x 8 tile of (bra pair, ket pair)
+// tasks; lane l of every wave holds quartet (p0 + l/8, q0 + l%8).  Classes
+// with many Cartesian components are cut into SPLIT slices over the leading
+// components (a, then b, then c); each wave evaluates whole slices, so the
+// slice index is wave-uniform and dispatched at compile time: every array
+// index stays a constant and the slice's integrals live in registers.  The
+// roots and the vertical recurrence are recomputed per slice.
+
+// Slice geometry: depth 0 = all components, 1 = fixed a, 2 = fixed (a, b),
+// 3 = fixed (a, b, c).
+template <int NA, int NB, int NC, int SPLIT>
+struct SliceShape {
+    static constexpr int depth = SPLIT == 1 ? 0 : SPLIT == NA ? 1 : SPLIT == NA * NB ? 2 : 3;
+};
+
+struct QuartetCtx {
+    int p, q, mol;
+    bool alive, pvalid, qvalid;
+    double4 rA, rC, AB, CD;
+    int pb0, npb, pk0, npk;
+    double fac;
+};
+
+// Sum over the 8 lanes sharing the bra pair (xor 1,2,4) or the ket pair
+// (xor 8,16,32) of the 8 x 8 lane tile.
+template <int M0>
+__device__ __forceinline__ double tile_sum(double v)
+{
+    v += __shfl_xor(v, M0);
+    v += __shfl_xor(v, 2 * M0);
+    v += __shfl_xor(v, 4 * M0);
+    return v;
+}
+
+template <int LA, int LB, int LC, int LD, int SPLIT, int S>
+__device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx& c)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
-    constexpr int NF = NA * NB * NC * ND;
     constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
     constexpr int NI = LA + LB + 1, NK = LC + LD + 1;
+    constexpr int DEPTH = SliceShape<NA, NB, NC, SPLIT>::depth;
+    // component ranges of this slice
+    constexpr int IA0 = DEPTH == 0 ? 0 : DEPTH == 1 ? S : DEPTH == 2 ? S / NB : S / (NB * NC);
+    constexpr int IA1 = DEPTH == 0 ? NA : IA0 + 1;
+    constexpr int IB0 = DEPTH <= 1 ? 0 : DEPTH == 2 ? S % NB : (S / NC) % NB;
+    constexpr int IB1 = DEPTH <= 1 ? NB : IB0 + 1;
+    constexpr int IC0 = DEPTH <= 2 ? 0 : S % NC;
+    constexpr int IC1 = DEPTH <= 2 ? NC : IC0 + 1;
+    constexpr int MA = IA1 - IA0, MB = IB1 - IB0, MC = IC1 - IC0;
+    constexpr int NFL = MA * MB * MC * ND;
 
-    const int bt = find_tile(a.bt_off, a.nbt, blockIdx.x);
-    const int kt = blockIdx.x - a.bt_off[bt];
-    const int4 t = a.bt[bt];
-    const int p = t.x + (threadIdx.x >> 3);
-    const int q = t.z + kt * TILE + (threadIdx.x & 7);
-    const int mol = a.bt_mol[bt];
-    bool alive = p < t.y && q < t.w && !(a.diag && q > p);
-    double4 rA = make_double4(0, 0, 0, 0), rC = rA;
-    int pb0 = 0, npb = 0, pk0 = 0, npk = 0;
-    if (alive) {
-        rA = a.P.rA[p];
-        rC = a.P.rA[q];
-        if (a.screen) alive = keep_quartet(a, mol, p, q, rA.w * rC.w);
-    }
-    if (alive) {
-        pb0 = a.P.prim0[p];
-        npb = a.P.nprim[p];
-        pk0 = a.P.prim0[q];
-        npk = a.P.nprim[q];
-    }
-    count_work(a.counters, alive, npb * npk);
-    if (!alive) return;
-    const double4 AB = a.P.rAB[p];
-    const double4 CD = a.P.rAB[q];
-
-    double gout[NF];
+    double gout[NFL];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) gout[f] = 0.0;
+    for (int f = 0; f < NFL; ++f) gout[f] = 0.0;
 
-    for (int ib = 0; ib < npb; ++ib) {
-        const double4 Pp = a.P.ppP[pb0 + ib];
-        const double2 Kp = a.P.ppK[pb0 + ib];
-        const double PAx = Pp.x - rA.x, PAy = Pp.y - rA.y, PAz = Pp.z - rA.z;
-        for (int ik = 0; ik < npk; ++ik) {
-            const double4 Qp = a.P.ppP[pk0 + ik];
-            const double2 Kq = a.P.ppK[pk0 + ik];
+    for (int ib = 0; ib < c.npb; ++ib) {
+        const double4 Pp = a.P.ppP[c.pb0 + ib];
+        const double2 Kp = a.P.ppK[c.pb0 + ib];
+        const double PAx = Pp.x - c.rA.x, PAy = Pp.y - c.rA.y, PAz = Pp.z - c.rA.z;
+        for (int ik = 0; ik < c.npk; ++ik) {
+            const double4 Qp = a.P.ppP[c.pk0 + ik];
+            const double2 Kq = a.P.ppK[c.pk0 + ik];
             const double pq = Pp.w + Qp.w;
             const double ipq = 1.0 / pq;
             const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
             const double T = Pp.w * Qp.w * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
             const double pref = Kp.x * Kq.x * sqrt(ipq);
             const RysCursor<NR> rys(a.rys, T);
-            const double QCx = Qp.x - rC.x, QCy = Qp.y - rC.y, QCz = Qp.z - rC.z;
+            const double QCx = Qp.x - c.rC.x, QCy = Qp.y - c.rC.y, QCz = Qp.z - c.rC.z;
 #pragma unroll 1
             for (int r = 0; r < NR; ++r) {
                 double rx, rw;
@@ -219,18 +236,18 @@ __global__ __launch_bounds__(64) void k_jk_lane(const JKArgs a)
                 double Ix[LA + 1][LB + 1][LC + 1][LD + 1];
                 double Iy[LA + 1][LB + 1][LC + 1][LD + 1];
                 double Iz[LA + 1][LB + 1][LC + 1][LD + 1];
-                hrr<LA, LB, LC, LD>(gx, AB.x, CD.x, Ix);
-                hrr<LA, LB, LC, LD>(gy, AB.y, CD.y, Iy);
-                hrr<LA, LB, LC, LD>(gz, AB.z, CD.z, Iz);
+                hrr<LA, LB, LC, LD>(gx, c.AB.x, c.CD.x, Ix);
+                hrr<LA, LB, LC, LD>(gy, c.AB.y, c.CD.y, Iy);
+                hrr<LA, LB, LC, LD>(gz, c.AB.z, c.CD.z, Iz);
 #pragma unroll
-                for (int ia = 0; ia < NA; ++ia)
+                for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
-                for (int ib2 = 0; ib2 < NB; ++ib2)
+                for (int ib2 = IB0; ib2 < IB1; ++ib2)
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
+                for (int ic = IC0; ic < IC1; ++ic)
 #pragma unroll
                 for (int id = 0; id < ND; ++id) {
-                    const int f = ((ia * NB + ib2) * NC + ic) * ND + id;
+                    const int f = (((ia - IA0) * MB + ib2 - IB0) * MC + ic - IC0) * ND + id;
                     gout[f] = fma(Ix[cart_x(LA, ia)][cart_x(LB, ib2)][cart_x(LC, ic)][cart_x(LD, id)]
                                       * Iy[cart_y(LA, ia)][cart_y(LB, ib2)][cart_y(LC, ic)][cart_y(LD, id)],
                                   Iz[cart_z(LA, ia)][cart_z(LB, ib2)][cart_z(LC, ic)][cart_z(LD, id)],
@@ -239,141 +256,204 @@ __global__ __launch_bounds__(64) void k_jk_lane(const JKArgs a)
             }
         }
     }
-
-    // degeneracy of the unique quartet
-    double fac = 1.0;
-    if (a.P.shA[p] == a.P.shB[p]) fac *= 0.5;
-    if (a.P.shA[q] == a.P.shB[q]) fac *= 0.5;
-    if (p == q && a.diag) fac *= 0.5;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) gout[f] *= fac;
+    for (int f = 0; f < NFL; ++f) gout[f] *= c.fac;
+#define GOUT(ia, ib2, ic, id) gout[((((ia) - IA0) * MB + (ib2) - IB0) * MC + (ic) - IC0) * ND + (id)]
 
-    const long long base = a.M.dbase[mol];
-    const int ld = a.M.ld[mol];
-    const int oa = a.P.cartA[p], ob = a.P.cartB[p], oc = a.P.cartA[q], od = a.P.cartB[q];
+    const long long base = a.M.dbase[c.mol];
+    const int ld = a.M.ld[c.mol];
+    const int lane = threadIdx.x & 63;
+    // pair data of dead lanes: only valid pairs' offsets are read
+    const int oa = c.pvalid ? a.P.cartA[c.p] : 0, ob = c.pvalid ? a.P.cartB[c.p] : 0;
+    const int oc = c.qvalid ? a.P.cartA[c.q] : 0, od = c.qvalid ? a.P.cartB[c.q] : 0;
 
     for (int s = 0; s < a.n_j; ++s) {
         const double* __restrict__ D = a.Dj + s * a.slot + base;
         double* V = a.J + s * a.slot + base;
-        double dab[NA][NB], dcd[NC][ND];
+        // J'[c,d] += sum_ab v Ds[a,b]: summed over this slice's (a, b) and the 8 bra lanes
+        double dab[MA][MB];
 #pragma unroll
-        for (int i = 0; i < NA; ++i)
+        for (int i = 0; i < MA; ++i)
 #pragma unroll
-            for (int j = 0; j < NB; ++j) dab[i][j] = D[(oa + i) * ld + ob + j];
+            for (int j = 0; j < MB; ++j) dab[i][j] = c.alive ? D[(oa + IA0 + i) * ld + ob + IB0 + j] : 0.0;
 #pragma unroll
-        for (int i = 0; i < NC; ++i)
-#pragma unroll
-            for (int j = 0; j < ND; ++j) dcd[i][j] = D[(oc + i) * ld + od + j];
-#pragma unroll
-        for (int ic = 0; ic < NC; ++ic)
+        for (int ic = IC0; ic < IC1; ++ic)
 #pragma unroll
             for (int id = 0; id < ND; ++id) {
                 double acc = 0.0;
 #pragma unroll
-                for (int ia = 0; ia < NA; ++ia)
+                for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
-                    for (int ib2 = 0; ib2 < NB; ++ib2)
-                        acc = fma(gout[((ia * NB + ib2) * NC + ic) * ND + id], dab[ia][ib2], acc);
-                atomic_add(V + (oc + ic) * ld + od + id, acc);
+                    for (int ib2 = IB0; ib2 < IB1; ++ib2) acc = fma(GOUT(ia, ib2, ic, id), dab[ia - IA0][ib2 - IB0], acc);
+                acc = tile_sum<8>(acc);
+                if (lane < 8 && c.qvalid && acc != 0.0) atomic_add(V + (oc + ic) * ld + od + id, acc);
             }
+        // J'[a,b] += sum_cd v Ds[c,d]: summed over this slice's (c, d) and the 8 ket lanes
+        double dcd[MC][ND];
 #pragma unroll
-        for (int ia = 0; ia < NA; ++ia)
+        for (int i = 0; i < MC; ++i)
 #pragma unroll
-            for (int ib2 = 0; ib2 < NB; ++ib2) {
+            for (int j = 0; j < ND; ++j) dcd[i][j] = c.alive ? D[(oc + IC0 + i) * ld + od + j] : 0.0;
+#pragma unroll
+        for (int ia = IA0; ia < IA1; ++ia)
+#pragma unroll
+            for (int ib2 = IB0; ib2 < IB1; ++ib2) {
                 double acc = 0.0;
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic)
+                for (int ic = IC0; ic < IC1; ++ic)
 #pragma unroll
-                    for (int id = 0; id < ND; ++id)
-                        acc = fma(gout[((ia * NB + ib2) * NC + ic) * ND + id], dcd[ic][id], acc);
-                atomic_add(V + (oa + ia) * ld + ob + ib2, acc);
+                    for (int id = 0; id < ND; ++id) acc = fma(GOUT(ia, ib2, ic, id), dcd[ic - IC0][id], acc);
+                acc = tile_sum<1>(acc);
+                if ((lane & 7) == 0 && c.pvalid && acc != 0.0) atomic_add(V + (oa + ia) * ld + ob + ib2, acc);
             }
     }
 
+    if (!c.alive) return;
     for (int s = 0; s < a.n_k; ++s) {
         const double* __restrict__ D = a.Dk + s * a.slot + base;
         double* V = a.K + s * a.slot + base;
         // K'[a,d] += sum_bc v D[b,c]
         {
-            double dbc[NB][NC];
+            double dbc[MB][MC];
 #pragma unroll
-            for (int i = 0; i < NB; ++i)
+            for (int i = 0; i < MB; ++i)
 #pragma unroll
-                for (int j = 0; j < NC; ++j) dbc[i][j] = D[(ob + i) * ld + oc + j];
+                for (int j = 0; j < MC; ++j) dbc[i][j] = D[(ob + IB0 + i) * ld + oc + IC0 + j];
 #pragma unroll
-            for (int ia = 0; ia < NA; ++ia)
+            for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
                 for (int id = 0; id < ND; ++id) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int ib2 = 0; ib2 < NB; ++ib2)
+                    for (int ib2 = IB0; ib2 < IB1; ++ib2)
 #pragma unroll
-                        for (int ic = 0; ic < NC; ++ic)
-                            acc = fma(gout[((ia * NB + ib2) * NC + ic) * ND + id], dbc[ib2][ic], acc);
+                        for (int ic = IC0; ic < IC1; ++ic)
+                            acc = fma(GOUT(ia, ib2, ic, id), dbc[ib2 - IB0][ic - IC0], acc);
                     atomic_add(V + (oa + ia) * ld + od + id, acc);
                 }
         }
         // K'[b,d] += sum_ac v D[a,c]
         {
-            double dac[NA][NC];
+            double dac[MA][MC];
 #pragma unroll
-            for (int i = 0; i < NA; ++i)
+            for (int i = 0; i < MA; ++i)
 #pragma unroll
-                for (int j = 0; j < NC; ++j) dac[i][j] = D[(oa + i) * ld + oc + j];
+                for (int j = 0; j < MC; ++j) dac[i][j] = D[(oa + IA0 + i) * ld + oc + IC0 + j];
 #pragma unroll
-            for (int ib2 = 0; ib2 < NB; ++ib2)
+            for (int ib2 = IB0; ib2 < IB1; ++ib2)
 #pragma unroll
                 for (int id = 0; id < ND; ++id) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int ia = 0; ia < NA; ++ia)
+                    for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
-                        for (int ic = 0; ic < NC; ++ic)
-                            acc = fma(gout[((ia * NB + ib2) * NC + ic) * ND + id], dac[ia][ic], acc);
+                        for (int ic = IC0; ic < IC1; ++ic)
+                            acc = fma(GOUT(ia, ib2, ic, id), dac[ia - IA0][ic - IC0], acc);
                     atomic_add(V + (ob + ib2) * ld + od + id, acc);
                 }
         }
         // K'[a,c] += sum_bd v D[b,d]
         {
-            double dbd[NB][ND];
+            double dbd[MB][ND];
 #pragma unroll
-            for (int i = 0; i < NB; ++i)
+            for (int i = 0; i < MB; ++i)
 #pragma unroll
-                for (int j = 0; j < ND; ++j) dbd[i][j] = D[(ob + i) * ld + od + j];
+                for (int j = 0; j < ND; ++j) dbd[i][j] = D[(ob + IB0 + i) * ld + od + j];
 #pragma unroll
-            for (int ia = 0; ia < NA; ++ia)
+            for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic) {
+                for (int ic = IC0; ic < IC1; ++ic) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int ib2 = 0; ib2 < NB; ++ib2)
+                    for (int ib2 = IB0; ib2 < IB1; ++ib2)
 #pragma unroll
                         for (int id = 0; id < ND; ++id)
-                            acc = fma(gout[((ia * NB + ib2) * NC + ic) * ND + id], dbd[ib2][id], acc);
+                            acc = fma(GOUT(ia, ib2, ic, id), dbd[ib2 - IB0][id], acc);
                     atomic_add(V + (oa + ia) * ld + oc + ic, acc);
                 }
         }
         // K'[b,c] += sum_ad v D[a,d]
         {
-            double dad[NA][ND];
+            double dad[MA][ND];
 #pragma unroll
-            for (int i = 0; i < NA; ++i)
+            for (int i = 0; i < MA; ++i)
 #pragma unroll
-                for (int j = 0; j < ND; ++j) dad[i][j] = D[(oa + i) * ld + od + j];
+                for (int j = 0; j < ND; ++j) dad[i][j] = D[(oa + IA0 + i) * ld + od + j];
 #pragma unroll
-            for (int ib2 = 0; ib2 < NB; ++ib2)
+            for (int ib2 = IB0; ib2 < IB1; ++ib2)
 #pragma unroll
-                for (int ic = 0; ic < NC; ++ic) {
+                for (int ic = IC0; ic < IC1; ++ic) {
                     double acc = 0.0;
 #pragma unroll
-                    for (int ia = 0; ia < NA; ++ia)
+                    for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
                         for (int id = 0; id < ND; ++id)
-                            acc = fma(gout[((ia * NB + ib2) * NC + ic) * ND + id], dad[ia][id], acc);
+                            acc = fma(GOUT(ia, ib2, ic, id), dad[ia - IA0][id], acc);
                     atomic_add(V + (ob + ib2) * ld + oc + ic, acc);
                 }
         }
     }
+#undef GOUT
+}
+
+template <int LA, int LB, int LC, int LD, int SPLIT, int S>
+__global__ __launch_bounds__(64) void k_jk(const JKArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int bt = find_tile(a.bt_off, a.nbt, blockIdx.x);
+    const int kt = blockIdx.x - a.bt_off[bt];
+    const int4 t = a.bt[bt];
+    QuartetCtx c;
+    c.p = t.x + (lane >> 3);
+    c.q = t.z + kt * TILE + (lane & 7);
+    c.mol = a.bt_mol[bt];
+    c.pvalid = c.p < t.y;
+    c.qvalid = c.q < t.w;
+    c.alive = c.pvalid && c.qvalid && !(a.diag && c.q > c.p);
+    c.rA = c.rC = c.AB = c.CD = make_double4(0, 0, 0, 0);
+    c.pb0 = c.npb = c.pk0 = c.npk = 0;
+    if (c.alive) {
+        c.rA = a.P.rA[c.p];
+        c.rC = a.P.rA[c.q];
+        if (a.screen) c.alive = keep_quartet(a, c.mol, c.p, c.q, c.rA.w * c.rC.w);
+    }
+    if (c.alive) {
+        c.pb0 = a.P.prim0[c.p];
+        c.npb = a.P.nprim[c.p];
+        c.pk0 = a.P.prim0[c.q];
+        c.npk = a.P.nprim[c.q];
+        c.AB = a.P.rAB[c.p];
+        c.CD = a.P.rAB[c.q];
+        double f = 1.0;
+        if (a.P.shA[c.p] == a.P.shB[c.p]) f *= 0.5;
+        if (a.P.shA[c.q] == a.P.shB[c.q]) f *= 0.5;
+        if (c.p == c.q && a.diag) f *= 0.5;
+        c.fac = f;
+    } else {
+        c.fac = 0.0;
+    }
+    if (S == 0) count_work(a.counters, c.alive, c.npb * c.npk);
+    if (!__ballot(c.alive)) return;   // whole tile screened out (wave-uniform exit)
+    quartet_slice<LA, LB, LC, LD, SPLIT, S>(a, c);
+}
+
+// Number of slices for a class: keep <= ~54 components per lane.
+__host__ __device__ constexpr int jk_split(int la, int lb, int lc, int ld)
+{
+    const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    const int nf = na * nb * nc * nd;
+    if (nf <= 81) return 1;
+    if (nf / na <= 54) return na;
+    if (nf / (na * nb) <= 54) return na * nb;
+    return na * nb * nc;
+}
+
+template <int LA, int LB, int LC, int LD, int S>
+int launch_jk_slice(int nblocks, const JKArgs& a, hipStream_t st)
+{
+    constexpr int SPLIT = jk_split(LA, LB, LC, LD);
+    hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, SPLIT, S>), dim3(nblocks), dim3(64), 0, st, a);
+    return (int)hipGetLastError();
 }
 
 }  // namespace mjk

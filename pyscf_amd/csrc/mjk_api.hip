@@ -25,7 +25,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mjk.h"
+#include "mjk.h"
 #include "c2s.h"
 #include "jk_common.h"
 #include "rys_eval.h"
@@ -53,6 +53,17 @@ constexpr double EXPCUTOFF = 60.0;   // drop primitive pairs with exp(-ab/p AB^2
 constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8, PTR_COORD = 1, ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2,
               NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6;
 constexpr int NQCLASS_MAX = NPAIRCLASS * (NPAIRCLASS + 1) / 2;
+
+// Shell pairs of a class are grouped by their number of primitive pairs so
+// that every tile runs a uniform primitive loop (no lane divergence).
+constexpr int NBUCKET = 16;
+int prim_bucket(int n)
+{
+    static const int edge[NBUCKET - 1] = {1, 2, 3, 4, 6, 9, 12, 16, 20, 25, 32, 40, 50, 64, 100};
+    for (int i = 0; i < NBUCKET - 1; ++i)
+        if (n <= edge[i]) return i;
+    return NBUCKET - 1;
+}
 
 struct GShell {
     int l, nprim, orig;
@@ -180,7 +191,7 @@ struct mjk_basis {
     double2* ppK = nullptr;
     int *prim0 = nullptr, *nprim = nullptr, *shA = nullptr, *shB = nullptr, *cartA = nullptr, *cartB = nullptr;
     double4 *rA = nullptr, *rAB = nullptr;
-    std::vector<int> seg;            // [class * nmol + mol] -> first pair; size NPAIRCLASS*nmol + 1
+    std::vector<int> seg;            // [(class * nmol + mol) * NBUCKET + bucket] -> first pair
     std::vector<double> q;           // Schwarz bound per pair (sorted order)
     std::vector<int> h_shA, h_shB, h_nprim;
     // per molecule / per AO device arrays
@@ -242,11 +253,15 @@ int build_tiles(mjk_basis* b, double cutoff)
             std::vector<int4> bt;
             std::vector<int> btm, off(1, 0);
             long long total = 0;
-            for (int m = 0; m < nmol; ++m) {
-                const int x0 = b->seg[X * nmol + m], x1 = b->seg[X * nmol + m + 1];
-                const int y0 = b->seg[Y * nmol + m], y1 = b->seg[Y * nmol + m + 1];
-                if (x1 <= x0 || y1 <= y0) continue;
-                for (int p0 = x0; p0 < x1; p0 += TILE) {
+            for (int m = 0; m < nmol; ++m)
+            for (int bx = 0; bx < NBUCKET; ++bx) {
+                const size_t sx = ((size_t)X * nmol + m) * NBUCKET + bx;
+                const int x0 = b->seg[sx], x1 = b->seg[sx + 1];
+                for (int p0 = x0; p0 < x1; p0 += TILE)
+                for (int by = 0; by < NBUCKET; ++by) {
+                    const size_t sy = ((size_t)Y * nmol + m) * NBUCKET + by;
+                    const int y0 = b->seg[sy], y1 = b->seg[sy + 1];
+                    if (y1 <= y0) continue;
                     const int p1 = std::min(p0 + TILE, x1);
                     int qend = y1;
                     if (cutoff > 0) {
@@ -422,7 +437,7 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     b->nsh_tot = hf;
 
     // shell pairs, grouped by class then molecule
-    std::vector<std::vector<PairHost>> byseg((size_t)NPAIRCLASS * nmol);
+    std::vector<std::vector<PairHost>> byseg((size_t)NPAIRCLASS * nmol * NBUCKET);
     const double sq2pi54 = std::sqrt(2.0) * std::pow(M_PI, 1.25);
     for (int mi = 0; mi < nmol; ++mi) {
         const MolHost& m = b->mols[mi];
@@ -455,13 +470,14 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
                         ph.K.push_back(make_double2(cc * std::exp(-E) * sq2pi54 / p, 1.0 / p));
                     }
                 if (ph.P.empty()) continue;
-                byseg[(size_t)pair_class(sa.l, sbb.l) * nmol + mi].push_back(std::move(ph));
+                const int bk = prim_bucket((int)ph.P.size());
+                byseg[((size_t)pair_class(sa.l, sbb.l) * nmol + mi) * NBUCKET + bk].push_back(std::move(ph));
             }
         }
     }
     // flatten (unsorted) for the Schwarz kernel
     auto flatten = [&](std::vector<int>& order_seg_start) {
-        order_seg_start.assign((size_t)NPAIRCLASS * nmol + 1, 0);
+        order_seg_start.assign((size_t)NPAIRCLASS * nmol * NBUCKET + 1, 0);
         int n = 0;
         for (size_t s = 0; s < byseg.size(); ++s) {
             order_seg_start[s] = n;
@@ -525,7 +541,7 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     double* dq = nullptr;
     if ((rc = dalloc(ctx, sizeof(double) * std::max(np, 1), (void**)&dq))) return rc;
     for (int X = 0; X < NPAIRCLASS; ++X) {
-        const int p0 = seg0[(size_t)X * nmol], p1 = seg0[(size_t)(X + 1) * nmol];
+        const int p0 = seg0[(size_t)X * nmol * NBUCKET], p1 = seg0[(size_t)(X + 1) * nmol * NBUCKET];
         if (p1 <= p0) continue;
         int la = 0;
         while ((la + 1) * (la + 2) / 2 <= X) ++la;
@@ -758,7 +774,8 @@ int mjk_basis_q_cond(mjk_ctx* ctx, const mjk_basis* b, int mol, double* q_cond)
     const int n = m.nbas_orig;
     for (int i = 0; i < n * n; ++i) q_cond[i] = 1e-100;
     for (int X = 0; X < NPAIRCLASS; ++X) {
-        const int p0 = b->seg[(size_t)X * b->nmol + mol], p1 = b->seg[(size_t)X * b->nmol + mol + 1];
+        const int p0 = b->seg[((size_t)X * b->nmol + mol) * NBUCKET];
+        const int p1 = b->seg[((size_t)X * b->nmol + mol + 1) * NBUCKET];
         for (int p = p0; p < p1; ++p) {
             const int i = m.sh[b->h_shA[p]].orig, j = m.sh[b->h_shB[p]].orig;
             q_cond[i * n + j] = std::max(q_cond[i * n + j], b->q[p]);
