@@ -40,6 +40,7 @@ def parse():
     ap.add_argument('--workload', default='c2', choices=['c1', 'c2', 'c3'])
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
+    ap.add_argument('--fp32-tol', type=float, default=1e-7)
     return ap.parse_args()
 
 
@@ -105,51 +106,61 @@ def main():
         eng.check(L.mjk_device_alloc(ctx, nbytes, ctypes.byref(p)), 'device_alloc')
     eng.check(L.mjk_memcpy_h2d(ctx, d_dm, ptr(dm), nbytes), 'h2d')
 
-    def build(stats=None):
-        rc = L.mjk_get_jk_device(ctx, basis.handle, d_dm, 1, 1, 1, 1, tol, 0.0, d_vj, d_vk,
-                                 ctypes.byref(stats) if stats is not None else None)
-        eng.check(rc, 'mjk_get_jk_device')
-
-    st = lib.mjk_stats()
-    build(st)                      # first call: tile lists for this cutoff, work buffers
-    for _ in range(args.warmup):
-        build()
-    eng.check(L.mjk_synchronize(ctx), 'sync')
-    # timed region: per-class HIP events on the launch stream, every step synchronised
-    eng.check(L.mjk_profile(ctx, basis.handle, 1), 'profile')
-    lat = []
-    eng.check(L.mjk_synchronize(ctx), 'sync')
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t = time.perf_counter()
-        build()
+    def timed(fp32_tol):
+        """warmup, then args.steps synchronised builds with per-class HIP events."""
+        st = lib.mjk_stats()
+        def build(stats=None):
+            rc = L.mjk_get_jk_device(ctx, basis.handle, d_dm, 1, 1, 1, 1, tol, fp32_tol, d_vj, d_vk,
+                                     ctypes.byref(stats) if stats is not None else None)
+            eng.check(rc, 'mjk_get_jk_device')
+        build(st)                      # first call: tile lists for this cutoff, work buffers
+        for _ in range(args.warmup):
+            build()
         eng.check(L.mjk_synchronize(ctx), 'sync')
-        lat.append(time.perf_counter() - t)
-    eng.check(L.mjk_synchronize(ctx), 'sync')
-    t_total = time.perf_counter() - t_start
-    maxc = 64
-    cls = numpy.zeros(4 * maxc, dtype=numpy.int32)
-    ms = numpy.zeros(maxc)
-    nq = numpy.zeros(maxc, dtype=numpy.int64)
-    npq = numpy.zeros(maxc, dtype=numpy.int64)
-    fl = numpy.zeros(maxc)
-    nl = numpy.zeros(maxc, dtype=numpy.int64)
-    k = L.mjk_profile_read(ctx, basis.handle, maxc, ptr(cls), ptr(ms), ptr(nq), ptr(npq), ptr(fl), ptr(nl))
-    L.mjk_profile(ctx, basis.handle, 0)
-    classes = []
-    for i in range(k):
-        classes.append({'class': '(%d%d|%d%d)' % tuple(cls[4 * i:4 * i + 4]), 'ms_per_launch': ms[i] / nl[i],
-                        'quartets': int(nq[i] // nl[i]), 'flops': fl[i] / nl[i],
-                        'tflops': fl[i] / (ms[i] * 1e-3) / 1e12 if ms[i] > 0 else 0.0})
-    classes.sort(key=lambda c: -c['ms_per_launch'])
-    dom = classes[0]
-    vj = numpy.empty_like(dm)
-    vk = numpy.empty_like(dm)
-    eng.check(L.mjk_memcpy_d2h(ctx, ptr(vj), d_vj, nbytes), 'd2h')
-    eng.check(L.mjk_memcpy_d2h(ctx, ptr(vk), d_vk, nbytes), 'd2h')
+        eng.check(L.mjk_profile(ctx, basis.handle, 1), 'profile')
+        lat = []
+        eng.check(L.mjk_synchronize(ctx), 'sync')
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            t = time.perf_counter()
+            build()
+            eng.check(L.mjk_synchronize(ctx), 'sync')
+            lat.append(time.perf_counter() - t)
+        eng.check(L.mjk_synchronize(ctx), 'sync')
+        t_total = time.perf_counter() - t_start
+        maxc = 64
+        cls = numpy.zeros(4 * maxc, dtype=numpy.int32)
+        ms = numpy.zeros(maxc)
+        nq = numpy.zeros(maxc, dtype=numpy.int64)
+        npq = numpy.zeros(maxc, dtype=numpy.int64)
+        fl = numpy.zeros(maxc)
+        nl = numpy.zeros(maxc, dtype=numpy.int64)
+        k = L.mjk_profile_read(ctx, basis.handle, maxc, ptr(cls), ptr(ms), ptr(nq), ptr(npq), ptr(fl), ptr(nl))
+        L.mjk_profile(ctx, basis.handle, 0)
+        classes = []
+        for i in range(k):
+            classes.append({'class': '(%d%d|%d%d)' % tuple(cls[4 * i:4 * i + 4]), 'ms_per_launch': ms[i] / nl[i],
+                            'quartets': int(nq[i] // nl[i]), 'flops': fl[i] / nl[i],
+                            'tflops': fl[i] / (ms[i] * 1e-3) / 1e12 if ms[i] > 0 else 0.0})
+        classes.sort(key=lambda c: -c['ms_per_launch'])
+        vj = numpy.empty_like(dm)
+        vk = numpy.empty_like(dm)
+        eng.check(L.mjk_memcpy_d2h(ctx, ptr(vj), d_vj, nbytes), 'd2h')
+        eng.check(L.mjk_memcpy_d2h(ctx, ptr(vk), d_vk, nbytes), 'd2h')
+        return {'t_total': t_total, 'lat': lat, 'classes': classes, 'stats': st, 'vj': vj, 'vk': vk}
+
+    r64 = timed(0.0)
+    rmx = timed(args.fp32_tol) if args.fp32_tol > 0 else None
     for p in (d_dm, d_vj, d_vk):
         L.mjk_device_free(ctx, p)
-
+    # mixed-precision SCF energy
+    emx = None
+    if rmx is not None:
+        mfm = rhf.RHF(mol, direct_scf_tol=tol, fp32_tol=args.fp32_tol)
+        mfm.conv_tol = 1e-10
+        emx = mfm.kernel()
+    t_total, lat, classes, st = r64['t_total'], r64['lat'], r64['classes'], r64['stats']
+    dom = classes[0]
     ms_step = t_total / args.steps * 1e3
     builds = args.steps / t_total
     kernel_ms = sum(c['ms_per_launch'] for c in classes)
@@ -176,12 +187,27 @@ def main():
         'scf': {'e_tot': e64, 'cycles': mf.cycles, 'converged': mf.converged, 'seconds': t_scf},
         'gpu_kernel_ms_per_build': kernel_ms,
         'fp64_tflops_all_classes': total_flops / (kernel_ms * 1e-3) / 1e12,
-        'roofline': {'bound': 'fp64', 'kernel': 'k_jk_lane' + dom['class'],
+        'roofline': {'bound': 'fp64', 'kernel': 'k_jk' + dom['class'],
                      'achieved': dom['tflops'], 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': dom['tflops'] * 1e12 / FP64_PEAK, 'traffic': None,
                      'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch']},
         'classes': classes[:8],
     }
+    if rmx is not None:
+        b_mx = args.steps / rmx['t_total']
+        out['mixed'] = {
+            'fp32_tol': args.fp32_tol,
+            'jk_builds_per_s': b_mx,
+            'speedup_vs_fp64': b_mx / builds,
+            'quartets_per_s': rmx['stats'].quartets * b_mx,
+            'fp32_quartet_fraction': rmx['stats'].quartets_fp32 / max(rmx['stats'].quartets, 1),
+            'p50_ms': float(numpy.percentile(rmx['lat'], 50) * 1e3),
+            'p99_ms': float(numpy.percentile(rmx['lat'], 99) * 1e3),
+            'max_abs_dJ': float(abs(rmx['vj'] - r64['vj']).max()),
+            'max_abs_dK': float(abs(rmx['vk'] - r64['vk']).max()),
+            'e_tot': emx,
+            'dE_vs_fp64': emx - e64,
+        }
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(mol, dm, tol, args.cpu_seconds)
     print(json.dumps(out))

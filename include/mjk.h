@@ -60,6 +60,8 @@ typedef struct {
     long long quartets;         /* evaluated (post-screening) shell quartets */
     long long prim_quartets;    /* evaluated primitive quartets */
     double flops;               /* algorithmic FLOPs of the evaluated quartets */
+    long long quartets_fp32;    /* of which evaluated in FP32 (mixed precision) */
+    double flops_fp32;          /* algorithmic FLOPs of the FP32 quartets */
 } mjk_stats;
 
 /* Context on one HIP device; hbm_budget = 0 means 80% of free memory.
@@ -99,8 +101,10 @@ int mjk_basis_q_cond(mjk_ctx* ctx, const mjk_basis* basis, int mol, double* q_co
  * _vhf.direct -> CVHFnr_direct_drv(CVHFdot_nrs8, CVHFnrs8_ji_s2kl,
  * CVHFnrs8_li_s2kj|s1kj) call chain (pyscf/scf/_vhf.py:405-464,
  * pyscf/lib/vhf/nr_direct.c:355-476), including hermi_triu.
- * fp32_tol > 0 enables mixed precision (quartets whose bound q_ij q_kl
- * max|D| < fp32_tol are evaluated in FP32); 0 keeps every quartet in FP64. */
+ * fp32_tol > 0 enables mixed precision: a wavefront of 64 quartets whose
+ * bounds q_ij q_kl max|D| (the prescreen's density factor) are all below
+ * fp32_tol evaluates its recurrences in FP32; Rys roots, the contracted
+ * block's digestion and all J/K accumulation stay FP64.  0: all FP64. */
 int mjk_get_jk(mjk_ctx* ctx, mjk_basis* basis, const double* dms, int n_dm, int hermi,
                int with_j, int with_k, double direct_scf_tol, double fp32_tol,
                double* vj, double* vk, mjk_stats* stats);
@@ -128,6 +132,8 @@ int mjk_device_free(mjk_ctx* ctx, void* ptr);
 int mjk_memcpy_h2d(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
 int mjk_memcpy_d2h(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
 int mjk_synchronize(mjk_ctx* ctx);
+/* Free and total HBM of a device (admission control of the request loop). */
+int mjk_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
 
 /* One-electron integrals for the SCF caller, computed on the host:
  * which = 0 overlap (int1e_ovlp), 1 kinetic (int1e_kin), 2 nuclear

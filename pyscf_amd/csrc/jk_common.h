@@ -67,8 +67,9 @@ struct JKArgs {
     const double* dm_cond;  // per molecule (nsh x nsh)
     double cutoff;
     int screen;
+    double fp32_tol;        // > 0: waves whose quartets all have q_ij q_kl dmax < fp32_tol run in FP32
     const double* rys;      // Rys table buffer
-    unsigned long long* counters;  // {evaluated quartets, primitive quartets}
+    unsigned long long* counters;  // {FP64 quartets, FP64 primitive quartets, FP32 quartets, FP32 primitive quartets}
 };
 
 struct QCondArgs {

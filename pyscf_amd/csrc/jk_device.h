@@ -49,35 +49,34 @@ __host__ __device__ constexpr int cart_y(int l, int i)
 __host__ __device__ constexpr int cart_z(int l, int i) { return l - cart_x(l, i) - cart_y(l, i); }
 
 // Vertical recurrence for one Cartesian direction and one root.
-template <int NI, int NK>
-__device__ __forceinline__ void vrr(double (&g)[NI][NK], double g00, double C, double Cp,
-                                    double B10, double B01, double B00)
+template <int NI, int NK, typename R>
+__device__ __forceinline__ void vrr(R (&g)[NI][NK], R g00, R C, R Cp, R B10, R B01, R B00)
 {
     g[0][0] = g00;
     if constexpr (NI > 1) {
         g[1][0] = C * g00;
 #pragma unroll
-        for (int i = 1; i < NI - 1; ++i) g[i + 1][0] = C * g[i][0] + (i * B10) * g[i - 1][0];
+        for (int i = 1; i < NI - 1; ++i) g[i + 1][0] = C * g[i][0] + (R(i) * B10) * g[i - 1][0];
     }
 #pragma unroll
     for (int k = 0; k < NK - 1; ++k) {
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
-            double v = Cp * g[i][k];
-            if (k > 0) v += (k * B01) * g[i][k - 1];
-            if (i > 0) v += (i * B00) * g[i - 1][k];
+            R v = Cp * g[i][k];
+            if (k > 0) v += (R(k) * B01) * g[i][k - 1];
+            if (i > 0) v += (R(i) * B00) * g[i - 1][k];
             g[i][k + 1] = v;
         }
     }
 }
 
 // Horizontal (transfer) recurrence: g(a+b, c+d) -> I(a, b, c, d).
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void hrr(const double (&g)[LA + LB + 1][LC + LD + 1], double AB, double CD,
-                                    double (&I)[LA + 1][LB + 1][LC + 1][LD + 1])
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void hrr(const R (&g)[LA + LB + 1][LC + LD + 1], R AB, R CD,
+                                    R (&I)[LA + 1][LB + 1][LC + 1][LD + 1])
 {
     constexpr int NI = LA + LB + 1, NK = LC + LD + 1;
-    double h[NI][LB + 1][NK];
+    R h[NI][LB + 1][NK];
 #pragma unroll
     for (int a = 0; a < NI; ++a)
 #pragma unroll
@@ -92,7 +91,7 @@ __device__ __forceinline__ void hrr(const double (&g)[LA + LB + 1][LC + LD + 1],
     for (int a = 0; a <= LA; ++a)
 #pragma unroll
         for (int b = 0; b <= LB; ++b) {
-            double e[NK][LD + 1];
+            R e[NK][LD + 1];
 #pragma unroll
             for (int k = 0; k < NK; ++k) e[k][0] = h[a][b][k];
 #pragma unroll
@@ -124,16 +123,18 @@ __device__ __forceinline__ int find_tile(const int* __restrict__ off, int n, int
 
 // Reference prescreen predicate (pyscf/lib/vhf/optimizer.c:90-117) on the
 // shells of this quartet; symmetric under i<->j, k<->l and bra<->ket.
-__device__ __forceinline__ bool keep_quartet(const JKArgs& a, int mol, int p, int q, double qpq)
+// Largest density factor of the predicate: the quartet survives when
+// q_ij q_kl > cutoff and qpq * dmax > cutoff; qpq * dmax also bounds its
+// contribution to J and K (the mixed-precision criterion).
+__device__ __forceinline__ double quartet_dmax(const JKArgs& a, int mol, int p, int q)
 {
-    if (qpq <= a.cutoff) return false;
     const double* __restrict__ dc = a.dm_cond + a.M.cbase[mol];
     const int ns = a.M.nsh[mol];
     const int i = a.P.shA[p], j = a.P.shB[p], k = a.P.shA[q], l = a.P.shB[q];
     double d = 4.0 * fmax(dc[j * ns + i], dc[l * ns + k]);
     d = fmax(d, fmax(dc[j * ns + k], dc[j * ns + l]));
     d = fmax(d, fmax(dc[i * ns + k], dc[i * ns + l]));
-    return d * qpq > a.cutoff;
+    return d;
 }
 
 // Wave-level tally of evaluated quartets and primitive quartets (one atomic
@@ -185,7 +186,7 @@ __device__ __forceinline__ double tile_sum(double v)
     return v;
 }
 
-template <int LA, int LB, int LC, int LD, int SPLIT, int S>
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R>
 __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx& c)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -202,9 +203,9 @@ __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx&
     constexpr int MA = IA1 - IA0, MB = IB1 - IB0, MC = IC1 - IC0;
     constexpr int NFL = MA * MB * MC * ND;
 
-    double gout[NFL];
+    R gout[NFL];
 #pragma unroll
-    for (int f = 0; f < NFL; ++f) gout[f] = 0.0;
+    for (int f = 0; f < NFL; ++f) gout[f] = R(0);
 
     for (int ib = 0; ib < c.npb; ++ib) {
         const double4 Pp = a.P.ppP[c.pb0 + ib];
@@ -229,16 +230,17 @@ __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx&
                 const double B10 = 0.5 * Kp.y * (1.0 - u * Qp.w);
                 const double B01 = 0.5 * Kq.y * (1.0 - u * Pp.w);
                 const double uq = u * Qp.w, up = u * Pp.w;
-                double gx[NI][NK], gy[NI][NK], gz[NI][NK];
-                vrr<NI, NK>(gx, 1.0, PAx - uq * PQx, QCx + up * PQx, B10, B01, B00);
-                vrr<NI, NK>(gy, 1.0, PAy - uq * PQy, QCy + up * PQy, B10, B01, B00);
-                vrr<NI, NK>(gz, rw * pref, PAz - uq * PQz, QCz + up * PQz, B10, B01, B00);
-                double Ix[LA + 1][LB + 1][LC + 1][LD + 1];
-                double Iy[LA + 1][LB + 1][LC + 1][LD + 1];
-                double Iz[LA + 1][LB + 1][LC + 1][LD + 1];
-                hrr<LA, LB, LC, LD>(gx, c.AB.x, c.CD.x, Ix);
-                hrr<LA, LB, LC, LD>(gy, c.AB.y, c.CD.y, Iy);
-                hrr<LA, LB, LC, LD>(gz, c.AB.z, c.CD.z, Iz);
+                R gx[NI][NK], gy[NI][NK], gz[NI][NK];
+                const R b10 = R(B10), b01 = R(B01), b00 = R(B00);
+                vrr<NI, NK, R>(gx, R(1), R(PAx - uq * PQx), R(QCx + up * PQx), b10, b01, b00);
+                vrr<NI, NK, R>(gy, R(1), R(PAy - uq * PQy), R(QCy + up * PQy), b10, b01, b00);
+                vrr<NI, NK, R>(gz, R(rw * pref), R(PAz - uq * PQz), R(QCz + up * PQz), b10, b01, b00);
+                R Ix[LA + 1][LB + 1][LC + 1][LD + 1];
+                R Iy[LA + 1][LB + 1][LC + 1][LD + 1];
+                R Iz[LA + 1][LB + 1][LC + 1][LD + 1];
+                hrr<LA, LB, LC, LD, R>(gx, R(c.AB.x), R(c.CD.x), Ix);
+                hrr<LA, LB, LC, LD, R>(gy, R(c.AB.y), R(c.CD.y), Iy);
+                hrr<LA, LB, LC, LD, R>(gz, R(c.AB.z), R(c.CD.z), Iz);
 #pragma unroll
                 for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
@@ -256,9 +258,11 @@ __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx&
             }
         }
     }
+    // contracted block in FP64 from here on (J/K accumulation is always FP64)
+    double gd[NFL];
 #pragma unroll
-    for (int f = 0; f < NFL; ++f) gout[f] *= c.fac;
-#define GOUT(ia, ib2, ic, id) gout[((((ia) - IA0) * MB + (ib2) - IB0) * MC + (ic) - IC0) * ND + (id)]
+    for (int f = 0; f < NFL; ++f) gd[f] = double(gout[f]) * c.fac;
+#define GOUT(ia, ib2, ic, id) gd[((((ia) - IA0) * MB + (ib2) - IB0) * MC + (ic) - IC0) * ND + (id)]
 
     const long long base = a.M.dbase[c.mol];
     const int ld = a.M.ld[c.mol];
@@ -412,10 +416,18 @@ __global__ __launch_bounds__(64) void k_jk(const JKArgs a)
     c.alive = c.pvalid && c.qvalid && !(a.diag && c.q > c.p);
     c.rA = c.rC = c.AB = c.CD = make_double4(0, 0, 0, 0);
     c.pb0 = c.npb = c.pk0 = c.npk = 0;
+    bool small = true;
     if (c.alive) {
         c.rA = a.P.rA[c.p];
         c.rC = a.P.rA[c.q];
-        if (a.screen) c.alive = keep_quartet(a, c.mol, c.p, c.q, c.rA.w * c.rC.w);
+        const double qpq = c.rA.w * c.rC.w;
+        if (a.screen || a.fp32_tol > 0) {
+            const double dmax = quartet_dmax(a, c.mol, c.p, c.q);
+            if (a.screen) c.alive = qpq > a.cutoff && dmax * qpq > a.cutoff;
+            small = dmax * qpq < a.fp32_tol;
+        } else {
+            small = false;
+        }
     }
     if (c.alive) {
         c.pb0 = a.P.prim0[c.p];
@@ -432,9 +444,286 @@ __global__ __launch_bounds__(64) void k_jk(const JKArgs a)
     } else {
         c.fac = 0.0;
     }
-    if (S == 0) count_work(a.counters, c.alive, c.npb * c.npk);
     if (!__ballot(c.alive)) return;   // whole tile screened out (wave-uniform exit)
-    quartet_slice<LA, LB, LC, LD, SPLIT, S>(a, c);
+    // FP32 only when every live quartet of the wave is below the bound
+    const bool use32 = a.fp32_tol > 0 && __ballot(c.alive && !small) == 0;
+    if (S == 0) count_work(a.counters + (use32 ? 2 : 0), c.alive, c.npb * c.npk);
+    if (use32) quartet_slice<LA, LB, LC, LD, SPLIT, S, float>(a, c);
+    else quartet_slice<LA, LB, LC, LD, SPLIT, S, double>(a, c);
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative kernel for classes with many Cartesian components.  A block of
+// 64 NW threads evaluates one quartet at a time (every barrier is
+// block-uniform); a block owns one bra pair of a tile and walks its 8 kets.
+// Per primitive quartet:
+//   phase A: item (root r, direction d) recomputes T and root r, runs the
+//            vertical recurrence and the transfer recurrence, streaming
+//            I_d(a, b, c, d') to LDS row by row;
+//   phase B: thread t accumulates components f = t, t + 64 NW, ... as
+//            sum_r Ix Iy Iz read from LDS.
+// Digestion: the contracted block goes to LDS (reusing the I region), the
+// D blocks are staged, and each J'/K' element is computed by one owner
+// thread and added with one FP64 atomic.
+template <int LA, int LB, int LC, int LD>
+struct CoopShape {
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int NF = NA * NB * NC * ND;
+    static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+    static constexpr int ISZ = (LA + 1) * (LB + 1) * (LC + 1) * (LD + 1);
+    static constexpr int SI = NR * 3 * ISZ;
+    static constexpr int SU = SI > NF ? SI : NF;
+    static constexpr int NDB = NA * NB + NC * ND + NB * NC + NA * NC + NB * ND + NA * ND;
+    static constexpr int NW = NF <= 64 * 24 ? 1 : NF <= 128 * 24 ? 2 : 4;
+    static constexpr int NT = 64 * NW;
+    static constexpr int CH = (NF + NT - 1) / NT;
+};
+
+// Transfer recurrence g(a+b, c+d) -> I(a, b, c, d) written to LDS as
+// dst[((a (LB+1) + b) (LC+1) + c) (LD+1) + d]; only two bra slabs live.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void hrr_to_lds(const double (&g)[LA + LB + 1][LC + LD + 1], double AB, double CD,
+                                           double* __restrict__ dst)
+{
+    constexpr int NI = LA + LB + 1, NK = LC + LD + 1;
+    double h[NI][NK];
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int k = 0; k < NK; ++k) h[i][k] = g[i][k];
+#pragma unroll
+    for (int b = 0; b <= LB; ++b) {
+        if (b > 0) {
+            // h_b(a) = h_{b-1}(a+1) + AB h_{b-1}(a), valid for a <= NI-1-b (ascending a: in place)
+#pragma unroll
+            for (int i = 0; i < NI - b; ++i)
+#pragma unroll
+                for (int k = 0; k < NK; ++k) h[i][k] = h[i + 1][k] + AB * h[i][k];
+        }
+#pragma unroll
+        for (int ia = 0; ia <= LA; ++ia) {
+            double e[NK][LD + 1];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) e[k][0] = h[ia][k];
+#pragma unroll
+            for (int d = 1; d <= LD; ++d)
+#pragma unroll
+                for (int k = 0; k < NK - d; ++k) e[k][d] = e[k + 1][d - 1] + CD * e[k][d - 1];
+#pragma unroll
+            for (int c = 0; c <= LC; ++c)
+#pragma unroll
+                for (int d = 0; d <= LD; ++d)
+                    dst[((ia * (LB + 1) + b) * (LC + 1) + c) * (LD + 1) + d] = e[c][d];
+        }
+    }
+}
+
+__device__ __forceinline__ int cart_x_rt(int l, int i)
+{
+    int lx = l, k = i;
+    while (k > l - lx) { k -= l - lx + 1; --lx; }
+    return lx;
+}
+__device__ __forceinline__ int cart_y_rt(int l, int i)
+{
+    int lx = l, k = i;
+    while (k > l - lx) { k -= l - lx + 1; --lx; }
+    return l - lx - k;
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(const JKArgs a)
+{
+    using Sh = CoopShape<LA, LB, LC, LD>;
+    constexpr int NA = Sh::NA, NB = Sh::NB, NC = Sh::NC, ND = Sh::ND, NF = Sh::NF, NR = Sh::NR;
+    constexpr int NI = LA + LB + 1, NK = LC + LD + 1;
+    constexpr int ISZ = Sh::ISZ, NT = Sh::NT, CH = Sh::CH;
+    constexpr int NITEM = NR * 3;
+    __shared__ double lds[Sh::SU + Sh::NDB];
+    double* const sblk = lds;                 // I arrays, later the contracted block
+    double* const sD = lds + Sh::SU;          // staged D blocks
+    const int tid = threadIdx.x;
+
+    const int tb = blockIdx.x >> 3;
+    const int row = blockIdx.x & 7;
+    const int bt = find_tile(a.bt_off, a.nbt, tb);
+    const int kt = tb - a.bt_off[bt];
+    const int4 t = a.bt[bt];
+    const int p = t.x + row;
+    if (p >= t.y) return;
+    const int mol = a.bt_mol[bt];
+    const double4 rA = a.P.rA[p];
+    const double4 AB = a.P.rAB[p];
+    const int pb0 = a.P.prim0[p], npb = a.P.nprim[p];
+    const bool diagA = a.P.shA[p] == a.P.shB[p];
+    const int oa = a.P.cartA[p], ob = a.P.cartB[p];
+    const long long base = a.M.dbase[mol];
+    const int ld = a.M.ld[mol];
+
+    // LDS offsets of this thread's components in the three I arrays of a root
+    int ox[CH], oy[CH], oz[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        int f = tid + NT * c;
+        if (f >= NF) f = NF - 1;
+        const int id = f % ND, ic = (f / ND) % NC, ib = (f / (ND * NC)) % NB, ia = f / (ND * NC * NB);
+        const int xa = cart_x_rt(LA, ia), xb = cart_x_rt(LB, ib), xc = cart_x_rt(LC, ic), xd = cart_x_rt(LD, id);
+        const int ya = cart_y_rt(LA, ia), yb = cart_y_rt(LB, ib), yc = cart_y_rt(LC, ic), yd = cart_y_rt(LD, id);
+        const int za = LA - xa - ya, zb = LB - xb - yb, zc = LC - xc - yc, zd = LD - xd - yd;
+        ox[c] = ((xa * (LB + 1) + xb) * (LC + 1) + xc) * (LD + 1) + xd;
+        oy[c] = ISZ + ((ya * (LB + 1) + yb) * (LC + 1) + yc) * (LD + 1) + yd;
+        oz[c] = 2 * ISZ + ((za * (LB + 1) + zb) * (LC + 1) + zc) * (LD + 1) + zd;
+    }
+
+    for (int col = 0; col < TILE; ++col) {
+        const int q = t.z + kt * TILE + col;
+        if (q >= t.w || (a.diag && q > p)) break;
+        const double4 rC = a.P.rA[q];
+        if (a.screen) {
+            const double qpq = rA.w * rC.w;
+            if (!(qpq > a.cutoff && quartet_dmax(a, mol, p, q) * qpq > a.cutoff)) continue;   // block-uniform
+        }
+        const int pk0 = a.P.prim0[q], npk = a.P.nprim[q];
+        if (tid == 0 && a.counters) {
+            atomicAdd(a.counters, 1ULL);
+            atomicAdd(a.counters + 1, (unsigned long long)(npb * npk));
+        }
+        const double4 CD = a.P.rAB[q];
+        double gout[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) gout[c] = 0.0;
+
+        for (int ib = 0; ib < npb; ++ib) {
+            const double4 Pp = a.P.ppP[pb0 + ib];
+            const double2 Kp = a.P.ppK[pb0 + ib];
+            for (int ik = 0; ik < npk; ++ik) {
+                const double4 Qp = a.P.ppP[pk0 + ik];
+                const double2 Kq = a.P.ppK[pk0 + ik];
+                for (int it = tid; it < NITEM; it += NT) {
+                    const int r = it / 3;
+                    const int d = it - 3 * r;
+                    const double pq = Pp.w + Qp.w;
+                    const double ipq = 1.0 / pq;
+                    const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
+                    const double T = Pp.w * Qp.w * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
+                    const RysCursor<NR> rys(a.rys, T);
+                    double rx, rw;
+                    rys.root(r, rx, rw);
+                    const double u = rx * ipq;
+                    const double PQd = d == 0 ? PQx : d == 1 ? PQy : PQz;
+                    const double PAd = d == 0 ? Pp.x - rA.x : d == 1 ? Pp.y - rA.y : Pp.z - rA.z;
+                    const double QCd = d == 0 ? Qp.x - rC.x : d == 1 ? Qp.y - rC.y : Qp.z - rC.z;
+                    const double ABd = d == 0 ? AB.x : d == 1 ? AB.y : AB.z;
+                    const double CDd = d == 0 ? CD.x : d == 1 ? CD.y : CD.z;
+                    const double g00 = d == 2 ? rw * Kp.x * Kq.x * sqrt(ipq) : 1.0;
+                    double g[NI][NK];
+                    vrr<NI, NK, double>(g, g00, PAd - u * Qp.w * PQd, QCd + u * Pp.w * PQd,
+                                        0.5 * Kp.y * (1.0 - u * Qp.w), 0.5 * Kq.y * (1.0 - u * Pp.w), 0.5 * u);
+                    hrr_to_lds<LA, LB, LC, LD>(g, ABd, CDd, sblk + it * ISZ);
+                }
+                __syncthreads();
+#pragma unroll
+                for (int c = 0; c < CH; ++c) {
+                    double s2 = 0.0;
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) {
+                        const double* Ir = sblk + r * 3 * ISZ;
+                        s2 = fma(Ir[ox[c]] * Ir[oy[c]], Ir[oz[c]], s2);
+                    }
+                    gout[c] += s2;
+                }
+                __syncthreads();
+            }
+        }
+
+        // contracted block (with its degeneracy factor) to LDS
+        double fac = diagA ? 0.5 : 1.0;
+        if (a.P.shA[q] == a.P.shB[q]) fac *= 0.5;
+        if (p == q && a.diag) fac *= 0.5;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int f = tid + NT * c;
+            if (f < NF) sblk[f] = gout[c] * fac;
+        }
+        const int oc = a.P.cartA[q], od = a.P.cartB[q];
+#define BLK(ia, ib2, ic, id) sblk[(((ia) * NB + (ib2)) * NC + (ic)) * ND + (id)]
+        for (int s = 0; s < a.n_j; ++s) {
+            const double* __restrict__ D = a.Dj + s * a.slot + base;
+            double* V = a.J + s * a.slot + base;
+            double* dab = sD;
+            double* dcd = sD + NA * NB;
+            for (int e = tid; e < NA * NB; e += NT) dab[e] = D[(oa + e / NB) * ld + ob + e % NB];
+            for (int e = tid; e < NC * ND; e += NT) dcd[e] = D[(oc + e / ND) * ld + od + e % ND];
+            __syncthreads();
+            for (int e = tid; e < NC * ND; e += NT) {
+                const int ic = e / ND, id = e % ND;
+                double acc = 0.0;
+                for (int ia = 0; ia < NA; ++ia)
+                    for (int ib2 = 0; ib2 < NB; ++ib2) acc = fma(BLK(ia, ib2, ic, id), dab[ia * NB + ib2], acc);
+                atomic_add(V + (oc + ic) * ld + od + id, acc);
+            }
+            for (int e = tid; e < NA * NB; e += NT) {
+                const int ia = e / NB, ib2 = e % NB;
+                double acc = 0.0;
+                for (int ic = 0; ic < NC; ++ic)
+                    for (int id = 0; id < ND; ++id) acc = fma(BLK(ia, ib2, ic, id), dcd[ic * ND + id], acc);
+                atomic_add(V + (oa + ia) * ld + ob + ib2, acc);
+            }
+            __syncthreads();
+        }
+        for (int s = 0; s < a.n_k; ++s) {
+            const double* __restrict__ D = a.Dk + s * a.slot + base;
+            double* V = a.K + s * a.slot + base;
+            double* dbc = sD;
+            double* dac = dbc + NB * NC;
+            double* dbd = dac + NA * NC;
+            double* dad = dbd + NB * ND;
+            for (int e = tid; e < NB * NC; e += NT) dbc[e] = D[(ob + e / NC) * ld + oc + e % NC];
+            for (int e = tid; e < NA * NC; e += NT) dac[e] = D[(oa + e / NC) * ld + oc + e % NC];
+            for (int e = tid; e < NB * ND; e += NT) dbd[e] = D[(ob + e / ND) * ld + od + e % ND];
+            for (int e = tid; e < NA * ND; e += NT) dad[e] = D[(oa + e / ND) * ld + od + e % ND];
+            __syncthreads();
+            for (int e = tid; e < NA * ND; e += NT) {          // K'[a,d] += sum_bc v D[b,c]
+                const int ia = e / ND, id = e % ND;
+                double acc = 0.0;
+                for (int ib2 = 0; ib2 < NB; ++ib2)
+                    for (int ic = 0; ic < NC; ++ic) acc = fma(BLK(ia, ib2, ic, id), dbc[ib2 * NC + ic], acc);
+                atomic_add(V + (oa + ia) * ld + od + id, acc);
+            }
+            for (int e = tid; e < NB * ND; e += NT) {          // K'[b,d] += sum_ac v D[a,c]
+                const int ib2 = e / ND, id = e % ND;
+                double acc = 0.0;
+                for (int ia = 0; ia < NA; ++ia)
+                    for (int ic = 0; ic < NC; ++ic) acc = fma(BLK(ia, ib2, ic, id), dac[ia * NC + ic], acc);
+                atomic_add(V + (ob + ib2) * ld + od + id, acc);
+            }
+            for (int e = tid; e < NA * NC; e += NT) {          // K'[a,c] += sum_bd v D[b,d]
+                const int ia = e / NC, ic = e % NC;
+                double acc = 0.0;
+                for (int ib2 = 0; ib2 < NB; ++ib2)
+                    for (int id = 0; id < ND; ++id) acc = fma(BLK(ia, ib2, ic, id), dbd[ib2 * ND + id], acc);
+                atomic_add(V + (oa + ia) * ld + oc + ic, acc);
+            }
+            for (int e = tid; e < NB * NC; e += NT) {          // K'[b,c] += sum_ad v D[a,d]
+                const int ib2 = e / NC, ic = e % NC;
+                double acc = 0.0;
+                for (int ia = 0; ia < NA; ++ia)
+                    for (int id = 0; id < ND; ++id) acc = fma(BLK(ia, ib2, ic, id), dad[ia * ND + id], acc);
+                atomic_add(V + (ob + ib2) * ld + oc + ic, acc);
+            }
+            __syncthreads();
+        }
+#undef BLK
+        __syncthreads();   // the block region is rewritten by the next quartet
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+int launch_jk_coop(int nblocks, const JKArgs& a, hipStream_t st)
+{
+    constexpr int NT = CoopShape<LA, LB, LC, LD>::NT;
+    hipLaunchKernelGGL((k_jk_coop<LA, LB, LC, LD>), dim3(nblocks * TILE), dim3(NT), 0, st, a);
+    return (int)hipGetLastError();
 }
 
 // Number of slices for a class: keep <= ~54 components per lane.

@@ -229,7 +229,7 @@ int launch_qcond(int la, int lb, const QCondArgs& a, hipStream_t st)
     case A * 4 + B: \
         hipLaunchKernelGGL((k_qcond<A, B>), dim3(nb), dim3(64), 0, st, a); \
         return (int)hipGetLastError();
-        Q(0, 0) Q(1, 0) Q(1, 1) Q(2, 0) Q(2, 1) Q(2, 2)
+        Q(0, 0) Q(1, 0) Q(1, 1) Q(2, 0) Q(2, 1) Q(2, 2) Q(3, 0) Q(3, 1) Q(3, 2) Q(3, 3)
 #undef Q
     default:
         return -1;

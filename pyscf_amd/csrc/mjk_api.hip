@@ -639,7 +639,7 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     if ((rc = track_upload(b, hmol, &b->d_sh_mol))) return rc;
     if ((rc = track_upload(b, hs0, &b->d_sh_s0))) return rc;
     if ((rc = track_upload(b, hn, &b->d_sh_n))) return rc;
-    if ((rc = dalloc(ctx, sizeof(unsigned long long) * 2 * NQCLASS_MAX, (void**)&b->counters))) return rc;
+    if ((rc = dalloc(ctx, sizeof(unsigned long long) * 4 * NQCLASS_MAX, (void**)&b->counters))) return rc;
     b->owned.push_back(b->counters);
     return build_tiles(b, cutoff);
 }
@@ -793,7 +793,6 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     if (n_dm <= 0 || !d_dms) return fail(ctx, MJK_EINVAL, "n_dm must be positive");
     if (hermi < 0 || hermi > 2) return fail(ctx, MJK_EINVAL, "hermi must be 0, 1 or 2");
     if ((with_j && !d_vj) || (with_k && !d_vk)) return fail(ctx, MJK_EINVAL, "missing output buffer");
-    if (fp32_tol > 0) return fail(ctx, MJK_EUNSUPPORTED, "mixed precision is not built yet");
     if (!with_j && !with_k) return MJK_OK;
     std::lock_guard<std::recursive_mutex> lock(ctx->mu);
     auto t0 = std::chrono::steady_clock::now();
@@ -809,13 +808,14 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     if ((rc = ensure(ctx, &b->J, b->cap_j, (long long)std::max(n_j, 1) * b->slot_c))) return rc;
     if ((rc = ensure(ctx, &b->Dk, b->cap_dk, (long long)std::max(n_k, 1) * b->slot_c))) return rc;
     if ((rc = ensure(ctx, &b->K, b->cap_k, (long long)std::max(n_k, 1) * b->slot_c))) return rc;
-    if (screen && (rc = ensure(ctx, &b->dmcond, b->cap_cond, b->cond_total))) return rc;
+    const bool need_cond = screen || fp32_tol > 0;
+    if (need_cond && (rc = ensure(ctx, &b->dmcond, b->cap_cond, b->cond_total))) return rc;
     hipStream_t st = ctx->stream;
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
     if (n_j) HIPCHK(ctx, hipMemsetAsync(b->J, 0, sizeof(double) * n_j * b->slot_c, st));
     if (n_k) HIPCHK(ctx, hipMemsetAsync(b->K, 0, sizeof(double) * n_k * b->slot_c, st));
-    HIPCHK(ctx, hipMemsetAsync(b->counters, 0, sizeof(unsigned long long) * 2 * NQCLASS_MAX, st));
-    if (screen && launch_dmcond(dmcond_args(b), d_dms, b->slot_s, n_dm, b->dmcond, st))
+    HIPCHK(ctx, hipMemsetAsync(b->counters, 0, sizeof(unsigned long long) * 4 * NQCLASS_MAX, st));
+    if (need_cond && launch_dmcond(dmcond_args(b), d_dms, b->slot_s, n_dm, b->dmcond, st))
         return fail(ctx, MJK_EDEVICE, "dm_cond launch failed");
     if (launch_sph2cart(xform(b), b->maxld, d_dms, b->slot_s, n_dm, hermi, with_j, with_k, b->Dj, b->Dk,
                         b->slot_c, st))
@@ -837,6 +837,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.dm_cond = b->dmcond;
     a.cutoff = cut;
     a.screen = screen ? 1 : 0;
+    a.fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
     a.rys = ctx->rys;
     const bool prof = b->profiling && b->pev.size() >= 2 * b->qcls.size();
     for (size_t c = 0; c < b->qcls.size(); ++c) {
@@ -848,7 +849,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         a.bt_off = qc.bt_off;
         a.nbt = qc.nbt;
         a.diag = qc.X == qc.Y;
-        a.counters = b->counters + 2 * c;
+        a.counters = b->counters + 4 * c;
         if (launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
             return fail(ctx, MJK_EUNSUPPORTED, "no kernel for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
         if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c + 1], st));
@@ -860,7 +861,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
     if (prof) {
         HIPCHK(ctx, hipStreamSynchronize(st));
-        std::vector<unsigned long long> cnt(2 * NQCLASS_MAX);
+        std::vector<unsigned long long> cnt(4 * NQCLASS_MAX);
         HIPCHK(ctx, hipMemcpy(cnt.data(), b->counters, sizeof(unsigned long long) * cnt.size(),
                               hipMemcpyDeviceToHost));
         for (size_t c = 0; c < b->qcls.size(); ++c) {
@@ -868,8 +869,8 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
             float ms = 0;
             HIPCHK(ctx, hipEventElapsedTime(&ms, b->pev[2 * c], b->pev[2 * c + 1]));
             b->prof_ms[c] += ms;
-            b->prof_nq[c] += (long long)cnt[2 * c];
-            b->prof_np[c] += (long long)cnt[2 * c + 1];
+            b->prof_nq[c] += (long long)(cnt[4 * c] + cnt[4 * c + 2]);
+            b->prof_np[c] += (long long)(cnt[4 * c + 1] + cnt[4 * c + 3]);
             b->prof_launches[c] += 1;
         }
     }
@@ -879,7 +880,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         HIPCHK(ctx, hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
         HIPCHK(ctx, hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
         HIPCHK(ctx, hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
-        std::vector<unsigned long long> cnt(2 * NQCLASS_MAX);
+        std::vector<unsigned long long> cnt(4 * NQCLASS_MAX);
         HIPCHK(ctx, hipMemcpy(cnt.data(), b->counters, sizeof(unsigned long long) * cnt.size(),
                               hipMemcpyDeviceToHost));
         stats->ms_quartets = t12;
@@ -887,11 +888,17 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         stats->quartets = 0;
         stats->prim_quartets = 0;
         stats->flops = 0;
+        stats->quartets_fp32 = 0;
+        stats->flops_fp32 = 0;
         for (size_t c = 0; c < b->qcls.size(); ++c) {
             const QClass& qc = b->qcls[c];
-            stats->quartets += (long long)cnt[2 * c];
-            stats->prim_quartets += (long long)cnt[2 * c + 1];
-            stats->flops += flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)cnt[2 * c], (double)cnt[2 * c + 1]);
+            const double f64 = flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)cnt[4 * c], (double)cnt[4 * c + 1]);
+            const double f32 = flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)cnt[4 * c + 2], (double)cnt[4 * c + 3]);
+            stats->quartets += (long long)(cnt[4 * c] + cnt[4 * c + 2]);
+            stats->prim_quartets += (long long)(cnt[4 * c + 1] + cnt[4 * c + 3]);
+            stats->quartets_fp32 += (long long)cnt[4 * c + 2];
+            stats->flops += f64 + f32;
+            stats->flops_fp32 += f32;
         }
         stats->ms_total =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -998,6 +1005,17 @@ int mjk_memcpy_d2h(mjk_ctx* ctx, void* dst, const void* src, size_t bytes)
     HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MJK_OK;
+}
+
+int mjk_mem_info(int device, size_t* free_bytes, size_t* total_bytes)
+{
+    if (!free_bytes || !total_bytes) return MJK_EINVAL;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return MJK_EDEVICE;
+    if (hipSetDevice(device) != hipSuccess) return MJK_EDEVICE;
+    const hipError_t e = hipMemGetInfo(free_bytes, total_bytes);
+    (void)hipSetDevice(cur);
+    return e == hipSuccess ? MJK_OK : MJK_EDEVICE;
 }
 
 int mjk_synchronize(mjk_ctx* ctx)

@@ -21,7 +21,9 @@ class mjk_stats(ctypes.Structure):
                 ('ms_transform', ctypes.c_double),
                 ('quartets', ctypes.c_longlong),
                 ('prim_quartets', ctypes.c_longlong),
-                ('flops', ctypes.c_double)]
+                ('flops', ctypes.c_double),
+                ('quartets_fp32', ctypes.c_longlong),
+                ('flops_fp32', ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -54,6 +56,7 @@ def _declare(lib):
     lib.mjk_memcpy_h2d.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
     lib.mjk_memcpy_d2h.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
     lib.mjk_synchronize.argtypes = [c_void_p]
+    lib.mjk_mem_info.argtypes = [c_int, P(c_size_t), P(c_size_t)]
     lib.mjk_rys_roots.argtypes = [c_int, c_double, c_void_p, c_void_p]
     lib.mjk_profile.argtypes = [c_void_p, c_void_p, c_int]
     lib.mjk_profile_read.argtypes = [c_void_p, c_void_p, c_int] + [c_void_p] * 6

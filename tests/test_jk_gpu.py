@@ -125,3 +125,31 @@ def test_batch_equals_single(oracle):
         assert abs(vj[0, off:off + n * n].reshape(n, n) - j0).max() < TOL
         assert abs(vk[0, off:off + n * n].reshape(n, n) - k0).max() < TOL
         off += n * n
+
+
+def test_fshells_def2_tzvp(oracle):
+    # O: 5s3p2d1f -- every class up to (ff|ff) via the cooperative kernel
+    mol = gto.M(atom='O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587; O 2.5 0.3 0.4', basis='def2-tzvp')
+    nao = mol.nao_nr()
+    rng = numpy.random.default_rng(7)
+    c = rng.standard_normal((nao, 8))
+    dm = c @ c.T / 8
+    vj, vk = hf.get_jk(mol, dm, hermi=1)
+    j0, k0 = oracle.get_jk(mol, dm, hermi=1)
+    assert abs(vj - j0).max() < TOL and abs(vk - k0).max() < TOL
+    a = rng.standard_normal((nao, nao))
+    vj, vk = hf.get_jk(mol, a, hermi=0)
+    j0, k0 = oracle.get_jk(mol, a, hermi=0)
+    assert abs(vj - j0).max() < TOL * 10 and abs(vk - k0).max() < TOL * 10
+
+
+def test_cooperative_classes_dd(oracle):
+    # (dd|dd) and (dd|dp) run the cooperative LDS kernel
+    mol = gto.M(atom='O 0 0 0; O 0 0 1.2; O 1.1 0.2 0.3', basis='def2-svp')
+    nao = mol.nao_nr()
+    c = numpy.random.default_rng(11).standard_normal((nao, 12))
+    dm = c @ c.T / 12
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    vj, vk = hf.get_jk(mol, dm, vhfopt=opt)
+    j0, k0 = oracle.get_jk(mol, dm, vhfopt=oracle.VHFOpt(mol, 1e-13))
+    assert abs(vj - j0).max() < TOL and abs(vk - k0).max() < TOL
