@@ -37,7 +37,11 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='c2', choices=['c1', 'c2', 'c3'])
+    ap.add_argument('--workload', default='c2', choices=['c1', 'c2', 'c3', 'c4', 'c5'])
+    ap.add_argument('--nmol', type=int, default=256, help='c4: molecules in the batch')
+    ap.add_argument('--nreq', type=int, default=48, help='c5: requests in the stream')
+    ap.add_argument('--rate', type=float, default=20.0, help='c5: Poisson arrival rate (1/s)')
+    ap.add_argument('--max-ao', type=int, default=1500, help='c5: largest request (AOs)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--fp32-tol', type=float, default=1e-7)
@@ -76,12 +80,116 @@ def cpu_baseline(mol, dm, tol, seconds):
                       'unique shell quartets in %.2f s, scaled by quartet count' % (stride, n_s, n_total, dt)}
 
 
+def bench_batched(args):
+    """256 random-walk chains (def2-SVP): batched SCF (one launch per quartet
+    class and iteration for all molecules) vs the same J/K builds issued one
+    molecule at a time."""
+    from pyscf_amd import gto, workloads
+    from pyscf_amd.scf import _vhf, batch
+    mols = [gto.M(atom=workloads.random_chain([4, i]), basis='def2-svp') for i in range(args.nmol)]
+    naos = [m.nao_nr() for m in mols]
+    res = {}
+    for label, tol32 in (('fp64', 0.0), ('mixed', args.fp32_tol)):
+        b = batch.BatchRHF(mols, fp32_tol=tol32)
+        b.conv_tol = 1e-9
+        t0 = time.perf_counter()
+        e = b.kernel()
+        t_scf = time.perf_counter() - t0
+        t_jk = sum(b.build_times)
+        res[label] = {'e_tot': e, 'scf_s': t_scf, 'iterations': b.cycles, 'jk_s': t_jk,
+                      'converged': int(b.converged.sum()),
+                      'mol_builds_per_s': len(mols) * len(b.build_times) / t_jk}
+    # unbatched baseline: the FP64 J/K build of every molecule, one call each
+    dms = b.dms
+    eng = _vhf.get_engine()
+    opts = [_vhf.VHFOpt(m, direct_scf_tol=1e-13, engine=eng) for m in mols]
+    for o, d in zip(opts, dms):
+        o.basis.get_jk(d[None], 1, True, True, 1e-13)
+    t0 = time.perf_counter()
+    for o, d in zip(opts, dms):
+        o.basis.get_jk(d[None], 1, True, True, 1e-13)
+    t_single = time.perf_counter() - t0
+    single = len(mols) / t_single
+    de = numpy.abs(res['mixed']['e_tot'] - res['fp64']['e_tot'])
+    out = {
+        'metric': METRIC, 'value': res['mixed']['mol_builds_per_s'], 'unit': 'molecule JK builds/s',
+        'n_gpus': 1, 'steps': res['mixed']['iterations'], 'warmup': 0,
+        'ms_per_step': 1e3 * res['mixed']['jk_s'] / res['mixed']['iterations'], 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64+f32', 'data': 'synthetic random-walk chains',
+        'config': {'workload': '%d random-walk chains (3-12 heavy atoms, seeds [4,i]), def2-SVP, '
+                   '%d-%d AOs, batched RHF to 1e-9' % (len(mols), min(naos), max(naos))},
+        'fp64_batched_mol_builds_per_s': res['fp64']['mol_builds_per_s'],
+        'mixed_batched_mol_builds_per_s': res['mixed']['mol_builds_per_s'],
+        'unbatched_fp64_mol_builds_per_s': single,
+        'speedup_mixed_batched_vs_unbatched_fp64': res['mixed']['mol_builds_per_s'] / single,
+        'speedup_fp64_batched_vs_unbatched_fp64': res['fp64']['mol_builds_per_s'] / single,
+        'max_abs_dE_mixed_vs_fp64': float(de.max()),
+        'converged': [res['fp64']['converged'], res['mixed']['converged']],
+        'scf_seconds': [res['fp64']['scf_s'], res['mixed']['scf_s']],
+    }
+    print(json.dumps(out))
+
+
+def bench_serving(args):
+    """Poisson stream of get_jk and RHF requests (50-max AOs, lognormal;
+    70% def2-SVP, 30% def2-TZVP) through the continuous-batching server."""
+    from pyscf_amd import gto, workloads
+    from pyscf_amd.server import JKServer, Request
+    rng = numpy.random.default_rng(5)
+    reqs = []
+    for i in range(args.nreq):
+        basis = 'def2-svp' if rng.random() < 0.7 else 'def2-tzvp'
+        target = float(numpy.clip(rng.lognormal(numpy.log(200), 0.8), 50, args.max_ao))
+        per_heavy = 14 if basis == 'def2-svp' else 31
+        nheavy = max(2, int(round(target / (per_heavy + 8))))
+        mol = gto.M(atom=workloads.random_chain([5, i], nmin=nheavy, nmax=nheavy), basis=basis)
+        kind = 'rhf' if rng.random() < 0.5 else 'get_jk'
+        dm = None
+        if kind == 'get_jk':
+            c = rng.standard_normal((mol.nao_nr(), max(2, mol.nelectron // 2))) / 4
+            dm = c @ c.T
+        reqs.append((kind, mol, dm, basis))
+    gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
+    srv = JKServer(nslots=8)
+    t0 = time.perf_counter()
+    futs = []
+    for (kind, mol, dm, _), g in zip(reqs, gaps):
+        time.sleep(g)
+        futs.append(srv.submit(Request(kind, mol, dm)))
+    ok = 0
+    for f in futs:
+        try:
+            f.result(timeout=1200)
+            ok += 1
+        except Exception:
+            pass
+    t_all = time.perf_counter() - t0
+    srv.close()
+    lat = numpy.array([r.t_done - r.t_arrival for r in srv.finished])
+    naos = [m.nao_nr() for _, m, _, _ in reqs]
+    out = {
+        'metric': METRIC, 'value': ok / t_all, 'unit': 'requests/s', 'n_gpus': 1, 'steps': len(reqs),
+        'warmup': 0, 'ms_per_step': 1e3 * t_all / len(reqs), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic random-walk chains',
+        'config': {'workload': 'Poisson stream (rate %.1f/s, seed 5) of %d requests, 50/50 get_jk/RHF, '
+                   '70%% def2-SVP / 30%% def2-TZVP, %d-%d AOs, 8 slots, 80%% HBM budget'
+                   % (args.rate, len(reqs), min(naos), max(naos))},
+        'p50_ms': float(numpy.percentile(lat, 50) * 1e3), 'p99_ms': float(numpy.percentile(lat, 99) * 1e3),
+        'completed': ok,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md); '
               'got --gpus %d' % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == 'c4':
+        return bench_batched(args)
+    if args.workload == 'c5':
+        return bench_serving(args)
     from pyscf_amd import lib
     from pyscf_amd.scf import _vhf, rhf
 

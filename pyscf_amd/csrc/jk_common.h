@@ -64,6 +64,9 @@ struct JKArgs {
     long long slot;         // slot stride (elements)
     int n_j;
     int n_k;
+    int ncopy;              // private accumulator copies (one per XCD); the kernel picks XCC_ID % ncopy
+    long long jcopy;        // element stride between copies of J'
+    long long kcopy;        // element stride between copies of K'
     const double* dm_cond;  // per molecule (nsh x nsh)
     double cutoff;
     int screen;

@@ -105,6 +105,17 @@ __device__ __forceinline__ void hrr(const R (&g)[LA + LB + 1][LC + LD + 1], R AB
         }
 }
 
+// Index of the XCD (accelerator complex die) this wave runs on.  Each XCD
+// accumulates into its own copy of J'/K', so the cache lines that atomics hit
+// stay in that XCD's L2 instead of moving between dies; the copies are summed
+// by the output transform.  Correctness does not depend on the mapping (the
+// atomics are device-coherent), only the locality does.
+__device__ __forceinline__ int xcc_copy(int ncopy)
+{
+    const int id = __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11));   // HW_REG_XCC_ID[3:0]
+    return ncopy > 1 ? (id % ncopy) : 0;
+}
+
 __device__ __forceinline__ void atomic_add(double* p, double v)
 {
     unsafeAtomicAdd(p, v);
@@ -271,9 +282,10 @@ __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx&
     const int oa = c.pvalid ? a.P.cartA[c.p] : 0, ob = c.pvalid ? a.P.cartB[c.p] : 0;
     const int oc = c.qvalid ? a.P.cartA[c.q] : 0, od = c.qvalid ? a.P.cartB[c.q] : 0;
 
+    const int cp = xcc_copy(a.ncopy);
     for (int s = 0; s < a.n_j; ++s) {
         const double* __restrict__ D = a.Dj + s * a.slot + base;
-        double* V = a.J + s * a.slot + base;
+        double* V = a.J + cp * a.jcopy + s * a.slot + base;
         // J'[c,d] += sum_ab v Ds[a,b]: summed over this slice's (a, b) and the 8 bra lanes
         double dab[MA][MB];
 #pragma unroll
@@ -315,7 +327,7 @@ __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx&
     if (!c.alive) return;
     for (int s = 0; s < a.n_k; ++s) {
         const double* __restrict__ D = a.Dk + s * a.slot + base;
-        double* V = a.K + s * a.slot + base;
+        double* V = a.K + cp * a.kcopy + s * a.slot + base;
         // K'[a,d] += sum_bc v D[b,c]
         {
             double dbc[MB][MC];
@@ -646,10 +658,11 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
             if (f < NF) sblk[f] = gout[c] * fac;
         }
         const int oc = a.P.cartA[q], od = a.P.cartB[q];
+        const int cp = xcc_copy(a.ncopy);
 #define BLK(ia, ib2, ic, id) sblk[(((ia) * NB + (ib2)) * NC + (ic)) * ND + (id)]
         for (int s = 0; s < a.n_j; ++s) {
             const double* __restrict__ D = a.Dj + s * a.slot + base;
-            double* V = a.J + s * a.slot + base;
+            double* V = a.J + cp * a.jcopy + s * a.slot + base;
             double* dab = sD;
             double* dcd = sD + NA * NB;
             for (int e = tid; e < NA * NB; e += NT) dab[e] = D[(oa + e / NB) * ld + ob + e % NB];
@@ -673,7 +686,7 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
         }
         for (int s = 0; s < a.n_k; ++s) {
             const double* __restrict__ D = a.Dk + s * a.slot + base;
-            double* V = a.K + s * a.slot + base;
+            double* V = a.K + cp * a.kcopy + s * a.slot + base;
             double* dbc = sD;
             double* dac = dbc + NB * NC;
             double* dbd = dac + NA * NC;

@@ -149,7 +149,8 @@ __global__ __launch_bounds__(256) void k_cart2sph(const XformArgs x, const doubl
                                                   const double* __restrict__ Kc, long long cslot,
                                                   int n_dm, int hermi, int with_j, int with_k,
                                                   double* __restrict__ vj, double* __restrict__ vk,
-                                                  long long sslot)
+                                                  long long sslot, int ncopy, long long jcopy,
+                                                  long long kcopy)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int jj = blockIdx.y * blockDim.y + threadIdx.y;
@@ -177,12 +178,15 @@ __global__ __launch_bounds__(256) void k_cart2sph(const XformArgs x, const doubl
                 if (w == 0.0) continue;
                 const long long e1 = cb + (long long)(c0i + a1) * ldc + c0j + a2;
                 const long long e2 = cb + (long long)(c0j + a2) * ldc + c0i + a1;
-                if (with_j) tj = fma(w, Jc[s * cslot + e1] + Jc[s * cslot + e2], tj);
-                if (with_k) {
-                    if (hermi == 1)
-                        tk = fma(w, Kc[s * cslot + e1] + Kc[s * cslot + e2], tk);
-                    else
-                        tk = fma(w, Kc[(2 * s) * cslot + e1] + Kc[(2 * s + 1) * cslot + e2], tk);
+                for (int cp = 0; cp < ncopy; ++cp) {
+                    if (with_j) tj = fma(w, Jc[cp * jcopy + s * cslot + e1] + Jc[cp * jcopy + s * cslot + e2], tj);
+                    if (with_k) {
+                        const double* Kp = Kc + cp * kcopy;
+                        if (hermi == 1)
+                            tk = fma(w, Kp[s * cslot + e1] + Kp[s * cslot + e2], tk);
+                        else
+                            tk = fma(w, Kp[(2 * s) * cslot + e1] + Kp[(2 * s + 1) * cslot + e2], tk);
+                    }
                 }
             }
         }
@@ -254,12 +258,12 @@ int launch_sph2cart(const XformArgs& x, int maxld, const double* Dsph, long long
 
 int launch_cart2sph(const XformArgs& x, int maxnao, const double* Jc, const double* Kc, long long cslot,
                     int n_dm, int hermi, int with_j, int with_k, double* vj, double* vk, long long sslot,
-                    hipStream_t st)
+                    int ncopy, long long jcopy, long long kcopy, hipStream_t st)
 {
     dim3 blk(64, 4);
     dim3 grd((unsigned)((x.nsph_tot + 63) / 64), (unsigned)((maxnao + 3) / 4));
     hipLaunchKernelGGL(k_cart2sph, grd, blk, 0, st, x, Jc, Kc, cslot, n_dm, hermi, with_j, with_k, vj, vk,
-                       sslot);
+                       sslot, ncopy, jcopy, kcopy);
     return (int)hipGetLastError();
 }
 

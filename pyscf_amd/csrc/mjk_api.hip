@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -39,7 +40,7 @@ int launch_sph2cart(const XformArgs& x, int maxld, const double* Dsph, long long
                     hipStream_t st);
 int launch_cart2sph(const XformArgs& x, int maxnao, const double* Jc, const double* Kc, long long cslot,
                     int n_dm, int hermi, int with_j, int with_k, double* vj, double* vk, long long sslot,
-                    hipStream_t st);
+                    int ncopy, long long jcopy, long long kcopy, hipStream_t st);
 int launch_dmcond(const DmCondArgs& x, const double* Dsph, long long sslot, int n_dm, double* out,
                   hipStream_t st);
 int lmax_kernels();
@@ -114,6 +115,7 @@ double flops_class(int a, int b, int c, int d, double nq, double nprimq)
 
 struct mjk_ctx {
     int device = 0;
+    int ncopy = 8;        // private J'/K' accumulator copies (MI355X has 8 XCDs)
     hipStream_t stream = nullptr;
     size_t budget = 0, used = 0;
     std::string err;
@@ -676,6 +678,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     size_t fr = 0, tot = 0;
     (void)hipMemGetInfo(&fr, &tot);
     ctx->budget = hbm_budget ? hbm_budget : (size_t)(0.8 * (double)fr);
+    if (const char* nc = getenv("MJK_NCOPY")) ctx->ncopy = std::max(1, std::min(16, atoi(nc)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     std::vector<double> tab(RYS_BUF_SIZE);
     rys_fill_buffer(tab.data());
@@ -805,15 +808,18 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     const int n_j = with_j ? n_dm : 0;
     const int n_k = with_k ? (hermi == 1 ? n_dm : 2 * n_dm) : 0;
     if ((rc = ensure(ctx, &b->Dj, b->cap_dj, (long long)std::max(n_j, 1) * b->slot_c))) return rc;
-    if ((rc = ensure(ctx, &b->J, b->cap_j, (long long)std::max(n_j, 1) * b->slot_c))) return rc;
+    const int ncopy = ctx->ncopy;
+    const long long jcopy = (long long)std::max(n_j, 1) * b->slot_c;
+    const long long kcopy = (long long)std::max(n_k, 1) * b->slot_c;
+    if ((rc = ensure(ctx, &b->J, b->cap_j, ncopy * jcopy))) return rc;
     if ((rc = ensure(ctx, &b->Dk, b->cap_dk, (long long)std::max(n_k, 1) * b->slot_c))) return rc;
-    if ((rc = ensure(ctx, &b->K, b->cap_k, (long long)std::max(n_k, 1) * b->slot_c))) return rc;
+    if ((rc = ensure(ctx, &b->K, b->cap_k, ncopy * kcopy))) return rc;
     const bool need_cond = screen || fp32_tol > 0;
     if (need_cond && (rc = ensure(ctx, &b->dmcond, b->cap_cond, b->cond_total))) return rc;
     hipStream_t st = ctx->stream;
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
-    if (n_j) HIPCHK(ctx, hipMemsetAsync(b->J, 0, sizeof(double) * n_j * b->slot_c, st));
-    if (n_k) HIPCHK(ctx, hipMemsetAsync(b->K, 0, sizeof(double) * n_k * b->slot_c, st));
+    if (n_j) HIPCHK(ctx, hipMemsetAsync(b->J, 0, sizeof(double) * ncopy * jcopy, st));
+    if (n_k) HIPCHK(ctx, hipMemsetAsync(b->K, 0, sizeof(double) * ncopy * kcopy, st));
     HIPCHK(ctx, hipMemsetAsync(b->counters, 0, sizeof(unsigned long long) * 4 * NQCLASS_MAX, st));
     if (need_cond && launch_dmcond(dmcond_args(b), d_dms, b->slot_s, n_dm, b->dmcond, st))
         return fail(ctx, MJK_EDEVICE, "dm_cond launch failed");
@@ -834,6 +840,9 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.slot = b->slot_c;
     a.n_j = n_j;
     a.n_k = n_k;
+    a.ncopy = ncopy;
+    a.jcopy = jcopy;
+    a.kcopy = kcopy;
     a.dm_cond = b->dmcond;
     a.cutoff = cut;
     a.screen = screen ? 1 : 0;
@@ -856,7 +865,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
     if (launch_cart2sph(xform(b), b->maxnao, b->J, b->K, b->slot_c, n_dm, hermi, with_j, with_k, d_vj, d_vk,
-                        b->slot_s, st))
+                        b->slot_s, ncopy, jcopy, kcopy, st))
         return fail(ctx, MJK_EDEVICE, "output transform launch failed");
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
     if (prof) {

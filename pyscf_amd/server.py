@@ -73,7 +73,8 @@ class JKServer:
         self.direct_scf_tol = direct_scf_tol
         self.fp32_tol = fp32_tol
         self.conv_tol = conv_tol
-        self.engines = [_vhf.Engine(device, hbm_budget=self.budget) for _ in range(nslots)]
+        # admission enforces the budget; each slot context may use what it is admitted with
+        self.engines = [_vhf.Engine(device) for _ in range(nslots)]
         self.free_slots = list(range(nslots))
         self.pending = []
         self.in_use = 0
