@@ -132,6 +132,8 @@ int mjk_device_free(mjk_ctx* ctx, void* ptr);
 int mjk_memcpy_h2d(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
 int mjk_memcpy_d2h(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
 int mjk_synchronize(mjk_ctx* ctx);
+/* Device-to-device copy of `bytes` (a multiple of 8) with a plain HIP kernel. */
+int mjk_device_copy(mjk_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Free and total HBM of a device (admission control of the request loop). */
 int mjk_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
 

@@ -1,4 +1,5 @@
 // Schwarz-bound, density-screening and basis-transform kernels.
+#include <algorithm>
 #include "jk_device.h"
 
 // Schwarz bound of each shell pair: q = sqrt(max_{a,b} |(ab|ab)|) over the
@@ -221,8 +222,24 @@ __global__ __launch_bounds__(256) void k_dmcond(const DmCondArgs x, const double
 }
 
 
+// Plain device-to-device copy (also the reference kernel of known traffic
+// for calibrating the HBM byte counters).
+__global__ __launch_bounds__(256) void k_copy(const double* __restrict__ src, double* __restrict__ dst, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
+int launch_copy(const double* src, double* dst, long long n, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    const long long nb = std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(256), 0, st, src, dst, n);
+    return (int)hipGetLastError();
+}
+
 int launch_qcond(int la, int lb, const QCondArgs& a, hipStream_t st)
 {
     const int nb = (a.np + 63) / 64;

@@ -44,6 +44,7 @@ int launch_cart2sph(const XformArgs& x, int maxnao, const double* Jc, const doub
 int launch_dmcond(const DmCondArgs& x, const double* Dsph, long long sslot, int n_dm, double* out,
                   hipStream_t st);
 int lmax_kernels();
+int launch_copy(const double* src, double* dst, long long n, hipStream_t st);
 }  // namespace mjk
 
 using namespace mjk;
@@ -1012,6 +1013,15 @@ int mjk_memcpy_d2h(mjk_ctx* ctx, void* dst, const void* src, size_t bytes)
 {
     if (!ctx) return MJK_EINVAL;
     HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MJK_OK;
+}
+
+int mjk_device_copy(mjk_ctx* ctx, void* dst, const void* src, size_t bytes)
+{
+    if (!ctx || (bytes % sizeof(double)) != 0) return MJK_EINVAL;
+    if (launch_copy((const double*)src, (double*)dst, (long long)(bytes / sizeof(double)), ctx->stream))
+        return fail(ctx, MJK_EDEVICE, "copy launch failed");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return MJK_OK;
 }

@@ -57,6 +57,7 @@ def _declare(lib):
     lib.mjk_memcpy_d2h.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
     lib.mjk_synchronize.argtypes = [c_void_p]
     lib.mjk_mem_info.argtypes = [c_int, P(c_size_t), P(c_size_t)]
+    lib.mjk_device_copy.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
     lib.mjk_rys_roots.argtypes = [c_int, c_double, c_void_p, c_void_p]
     lib.mjk_profile.argtypes = [c_void_p, c_void_p, c_int]
     lib.mjk_profile_read.argtypes = [c_void_p, c_void_p, c_int] + [c_void_p] * 6
