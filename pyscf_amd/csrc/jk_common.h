@@ -19,6 +19,7 @@ namespace mjk {
 constexpr int LMAX = 3;                       // s, p, d, f
 constexpr int NPAIRCLASS = (LMAX + 1) * (LMAX + 2) / 2;
 constexpr int TILE = 8;                       // quartet tile: 8 bra x 8 ket pairs
+constexpr int ROW_MAXBUCKET = 32;             // primitive-count buckets a row-kernel block can hold
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 __host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
@@ -73,6 +74,12 @@ struct JKArgs {
     double fp32_tol;        // > 0: waves whose quartets all have q_ij q_kl dmax < fp32_tol run in FP32
     const double* rys;      // Rys table buffer
     unsigned long long* counters;  // {FP64 quartets, FP64 primitive quartets, FP32 quartets, FP32 primitive quartets}
+    // row kernel: one block per bra pair p = pbase + blockIdx.x against all kets of the ket class
+    int pbase, nbra;        // bra pairs pbase .. pbase + nbra - 1
+    int rsplit;             // blocks sharing the kets of one bra pair
+    const int* pmol;        // molecule of every pair
+    const int* kseg;        // ket class segment starts: [mol * nbucket + bucket], one past = end
+    int nbucket;
 };
 
 struct QCondArgs {

@@ -197,22 +197,33 @@ __device__ __forceinline__ double tile_sum(double v)
     return v;
 }
 
+// Component ranges of slice S of a class cut into SPLIT slices.
+template <int LA, int LB, int LC, int LD, int SPLIT, int S>
+struct SliceRange {
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int DEPTH = SliceShape<NA, NB, NC, SPLIT>::depth;
+    static constexpr int IA0 = DEPTH == 0 ? 0 : DEPTH == 1 ? S : DEPTH == 2 ? S / NB : S / (NB * NC);
+    static constexpr int IA1 = DEPTH == 0 ? NA : IA0 + 1;
+    static constexpr int IB0 = DEPTH <= 1 ? 0 : DEPTH == 2 ? S % NB : (S / NC) % NB;
+    static constexpr int IB1 = DEPTH <= 1 ? NB : IB0 + 1;
+    static constexpr int IC0 = DEPTH <= 2 ? 0 : S % NC;
+    static constexpr int IC1 = DEPTH <= 2 ? NC : IC0 + 1;
+    static constexpr int MA = IA1 - IA0, MB = IB1 - IB0, MC = IC1 - IC0;
+    static constexpr int NFL = MA * MB * MC * ND;
+};
+
+// Contracted Cartesian block of one quartet restricted to slice S, times
+// the quartet's degeneracy factor, in FP64 (recurrences in precision R).
 template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R>
-__device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx& c)
+__device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
+                                          double (&gd)[SliceRange<LA, LB, LC, LD, SPLIT, S>::NFL])
 {
-    constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    using SR = SliceRange<LA, LB, LC, LD, SPLIT, S>;
+    constexpr int ND = SR::ND;
     constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
     constexpr int NI = LA + LB + 1, NK = LC + LD + 1;
-    constexpr int DEPTH = SliceShape<NA, NB, NC, SPLIT>::depth;
-    // component ranges of this slice
-    constexpr int IA0 = DEPTH == 0 ? 0 : DEPTH == 1 ? S : DEPTH == 2 ? S / NB : S / (NB * NC);
-    constexpr int IA1 = DEPTH == 0 ? NA : IA0 + 1;
-    constexpr int IB0 = DEPTH <= 1 ? 0 : DEPTH == 2 ? S % NB : (S / NC) % NB;
-    constexpr int IB1 = DEPTH <= 1 ? NB : IB0 + 1;
-    constexpr int IC0 = DEPTH <= 2 ? 0 : S % NC;
-    constexpr int IC1 = DEPTH <= 2 ? NC : IC0 + 1;
-    constexpr int MA = IA1 - IA0, MB = IB1 - IB0, MC = IC1 - IC0;
-    constexpr int NFL = MA * MB * MC * ND;
+    constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
+    constexpr int MB = SR::MB, MC = SR::MC, NFL = SR::NFL;
 
     R gout[NFL];
 #pragma unroll
@@ -270,10 +281,22 @@ __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx&
         }
     }
     // contracted block in FP64 from here on (J/K accumulation is always FP64)
-    double gd[NFL];
 #pragma unroll
     for (int f = 0; f < NFL; ++f) gd[f] = double(gout[f]) * c.fac;
+}
+
 #define GOUT(ia, ib2, ic, id) gd[((((ia) - IA0) * MB + (ib2) - IB0) * MC + (ic) - IC0) * ND + (id)]
+
+// Tile digestion: J' summed across the lanes sharing a pair, K' per lane.
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R>
+__device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx& c)
+{
+    using SR = SliceRange<LA, LB, LC, LD, SPLIT, S>;
+    constexpr int ND = SR::ND;
+    constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
+    constexpr int MA = SR::MA, MB = SR::MB, MC = SR::MC;
+    double gd[SR::NFL];
+    eri_slice<LA, LB, LC, LD, SPLIT, S, R>(a, c, gd);
 
     const long long base = a.M.dbase[c.mol];
     const int ld = a.M.ld[c.mol];
@@ -409,7 +432,6 @@ __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx&
                 }
         }
     }
-#undef GOUT
 }
 
 template <int LA, int LB, int LC, int LD, int SPLIT, int S>
@@ -757,6 +779,299 @@ int launch_jk_slice(int nblocks, const JKArgs& a, hipStream_t st)
     hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, SPLIT, S>), dim3(nblocks), dim3(64), 0, st, a);
     return (int)hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Row kernel.  Every K' contribution of a quartet (ab|cd) lands in rows a
+// and b of K' (K'[a,d], K'[a,c], K'[b,d], K'[b,c]).  A block of 256 threads
+// owns one bra pair and a share (1/rsplit) of the surviving kets of the ket
+// class: the Schwarz prefix of every primitive bucket is found by binary
+// search, the prefixes are concatenated, and each wave takes 64 consecutive
+// kets at a time.  The block accumulates its K' rows and its J'[a,b] block
+// in LDS with LDS atomics and flushes the non-zero entries to HBM once; only
+// J'[c,d] is added per quartet.  LDS holds n_k (NA + NB) ld + n_j NA NB
+// doubles (dynamic shared memory).
+__device__ __forceinline__ void lds_add(double* p, double v)
+{
+    atomicAdd(p, v);
+}
+
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R>
+__device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c, double* kr, double* jr,
+                                           int krow, long long base, int ld, int oa, int ob, int cp)
+{
+    using SR = SliceRange<LA, LB, LC, LD, SPLIT, S>;
+    constexpr int NA = SR::NA, NB = SR::NB, ND = SR::ND;
+    constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
+    constexpr int MA = SR::MA, MB = SR::MB, MC = SR::MC;
+    double gd[SR::NFL];
+    eri_slice<LA, LB, LC, LD, SPLIT, S, R>(a, c, gd);
+    const int oc = c.qvalid ? a.P.cartA[c.q] : 0, od = c.qvalid ? a.P.cartB[c.q] : 0;
+    const int lane = threadIdx.x & 63;
+    for (int s = 0; s < a.n_j; ++s) {
+        const double* __restrict__ D = a.Dj + s * a.slot + base;
+        double* V = a.J + cp * a.jcopy + s * a.slot + base;
+        // J'[c,d] += sum_ab v Ds[a,b]  (per quartet, global)
+        double dab[MA][MB];
+#pragma unroll
+        for (int i = 0; i < MA; ++i)
+#pragma unroll
+            for (int j = 0; j < MB; ++j) dab[i][j] = D[(oa + IA0 + i) * ld + ob + IB0 + j];
+        if (c.alive) {
+#pragma unroll
+            for (int ic = IC0; ic < IC1; ++ic)
+#pragma unroll
+                for (int id = 0; id < ND; ++id) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int ia = IA0; ia < IA1; ++ia)
+#pragma unroll
+                        for (int ib2 = IB0; ib2 < IB1; ++ib2)
+                            acc = fma(GOUT(ia, ib2, ic, id), dab[ia - IA0][ib2 - IB0], acc);
+                    atomic_add(V + (oc + ic) * ld + od + id, acc);
+                }
+        }
+        // J'[a,b] += sum_cd v Ds[c,d]  (summed over the wave, then into LDS)
+        double dcd[MC][ND];
+#pragma unroll
+        for (int i = 0; i < MC; ++i)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) dcd[i][j] = c.alive ? D[(oc + IC0 + i) * ld + od + j] : 0.0;
+#pragma unroll
+        for (int ia = IA0; ia < IA1; ++ia)
+#pragma unroll
+            for (int ib2 = IB0; ib2 < IB1; ++ib2) {
+                double acc = 0.0;
+#pragma unroll
+                for (int ic = IC0; ic < IC1; ++ic)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc = fma(GOUT(ia, ib2, ic, id), dcd[ic - IC0][id], acc);
+                acc = tile_sum<1>(acc);
+                acc = tile_sum<8>(acc);
+                if (lane == 0 && acc != 0.0) lds_add(jr + s * NA * NB + ia * NB + ib2, acc);
+            }
+    }
+    if (!c.alive) return;
+    for (int s = 0; s < a.n_k; ++s) {
+        const double* __restrict__ D = a.Dk + s * a.slot + base;
+        double* KA = kr + s * krow;              // rows a: K'[a, :]
+        double* KB = KA + NA * ld;               // rows b: K'[b, :]
+        // K'[a,d] += sum_bc v D[b,c];  K'[b,d] += sum_ac v D[a,c]
+        // K'[a,c] += sum_bd v D[b,d];  K'[b,c] += sum_ad v D[a,d]
+        double dbc[MB][MC], dac[MA][MC], dbd[MB][ND], dad[MA][ND];
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+            for (int j = 0; j < MC; ++j) dbc[i][j] = D[(ob + IB0 + i) * ld + oc + IC0 + j];
+#pragma unroll
+        for (int i = 0; i < MA; ++i)
+#pragma unroll
+            for (int j = 0; j < MC; ++j) dac[i][j] = D[(oa + IA0 + i) * ld + oc + IC0 + j];
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) dbd[i][j] = D[(ob + IB0 + i) * ld + od + j];
+#pragma unroll
+        for (int i = 0; i < MA; ++i)
+#pragma unroll
+            for (int j = 0; j < ND; ++j) dad[i][j] = D[(oa + IA0 + i) * ld + od + j];
+#pragma unroll
+        for (int ia = IA0; ia < IA1; ++ia)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0.0;
+#pragma unroll
+                for (int ib2 = IB0; ib2 < IB1; ++ib2)
+#pragma unroll
+                    for (int ic = IC0; ic < IC1; ++ic) acc = fma(GOUT(ia, ib2, ic, id), dbc[ib2 - IB0][ic - IC0], acc);
+                lds_add(KA + ia * ld + od + id, acc);
+            }
+#pragma unroll
+        for (int ib2 = IB0; ib2 < IB1; ++ib2)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0.0;
+#pragma unroll
+                for (int ia = IA0; ia < IA1; ++ia)
+#pragma unroll
+                    for (int ic = IC0; ic < IC1; ++ic) acc = fma(GOUT(ia, ib2, ic, id), dac[ia - IA0][ic - IC0], acc);
+                lds_add(KB + ib2 * ld + od + id, acc);
+            }
+#pragma unroll
+        for (int ia = IA0; ia < IA1; ++ia)
+#pragma unroll
+            for (int ic = IC0; ic < IC1; ++ic) {
+                double acc = 0.0;
+#pragma unroll
+                for (int ib2 = IB0; ib2 < IB1; ++ib2)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc = fma(GOUT(ia, ib2, ic, id), dbd[ib2 - IB0][id], acc);
+                lds_add(KA + ia * ld + oc + ic, acc);
+            }
+#pragma unroll
+        for (int ib2 = IB0; ib2 < IB1; ++ib2)
+#pragma unroll
+            for (int ic = IC0; ic < IC1; ++ic) {
+                double acc = 0.0;
+#pragma unroll
+                for (int ia = IA0; ia < IA1; ++ia)
+#pragma unroll
+                    for (int id = 0; id < ND; ++id) acc = fma(GOUT(ia, ib2, ic, id), dad[ia - IA0][id], acc);
+                lds_add(KB + ib2 * ld + oc + ic, acc);
+            }
+    }
+}
+
+// Minimum waves per SIMD the compiler must allow for the row kernel of a
+// class (caps its VGPR budget at 512 / value).
+#ifndef MJK_ROW_WPE
+#define MJK_ROW_WPE 1
+#endif
+template <int LA, int LB, int LC, int LD>
+struct RowOccupancy {
+    static constexpr int value = MJK_ROW_WPE;
+};
+
+template <int LA, int LB, int LC, int LD, int SPLIT, int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowOccupancy<LA, LB, LC, LD>::value)))
+void k_jk_row(const JKArgs a)
+{
+    constexpr int NA = ncart(LA), NB = ncart(LB);
+    constexpr int NROW = NA + NB;
+    extern __shared__ double lds_rows[];
+    __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // block b: bra pair pbase + b % nbra, ket chunks part, part + rsplit, ... of it
+    const int p = a.pbase + (int)(blockIdx.x % (unsigned)a.nbra);
+    const int part = (int)(blockIdx.x / (unsigned)a.nbra);
+    const int mol = a.pmol[p];
+    const int ld = a.M.ld[mol];
+    const long long base = a.M.dbase[mol];
+    const int krow = NROW * ld;
+    double* kr = lds_rows;
+    double* jr = lds_rows + a.n_k * krow;
+    const int nz = a.n_k * krow + a.n_j * NA * NB;
+    for (int e = tid; e < nz; e += 256) lds_rows[e] = 0.0;
+    const double qp = a.P.rA[p].w;
+    // surviving ket range of every primitive bucket: Schwarz prefix of the
+    // (descending) segment, and q <= p on the diagonal class
+    if (tid < a.nbucket) {
+        const int ys = a.kseg[mol * a.nbucket + tid], ye = a.kseg[mol * a.nbucket + tid + 1];
+        int yend = ye;
+        if (a.screen) {
+            int lo = ys, hi = ye;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (a.P.rA[mid].w * qp > a.cutoff) lo = mid + 1; else hi = mid;
+            }
+            yend = lo;
+        }
+        if (a.diag && yend > p + 1) yend = p + 1;
+        s_start[tid] = ys;
+        s_pre[tid + 1] = yend > ys ? yend - ys : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        s_pre[0] = 0;
+        for (int k = 1; k <= a.nbucket; ++k) s_pre[k] += s_pre[k - 1];
+    }
+    __syncthreads();
+    const int total = s_pre[a.nbucket];
+
+    QuartetCtx c;
+    c.p = p;
+    c.mol = mol;
+    c.pvalid = true;
+    c.rA = a.P.rA[p];
+    c.AB = a.P.rAB[p];
+    c.pb0 = a.P.prim0[p];
+    c.npb = a.P.nprim[p];
+    const bool diagA = a.P.shA[p] == a.P.shB[p];
+    const int oa = a.P.cartA[p], ob = a.P.cartB[p];
+    const int cp = xcc_copy(a.ncopy);
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    for (int i0 = (part * 4 + wave) * 64; i0 < total; i0 += 256 * a.rsplit) {
+        const int idx = i0 + lane;
+        c.qvalid = idx < total;
+        c.q = 0;
+        if (c.qvalid) {
+            int lo = 0, hi = a.nbucket;   // bucket k with s_pre[k] <= idx < s_pre[k + 1]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= idx) lo = mid; else hi = mid;
+            }
+            c.q = s_start[lo] + idx - s_pre[lo];
+        }
+        c.alive = c.qvalid;
+        c.rC = c.CD = make_double4(0, 0, 0, 0);
+        c.pk0 = c.npk = 0;
+        bool small = true;
+        if (c.alive) {
+            c.rC = a.P.rA[c.q];
+            const double qpq = qp * c.rC.w;
+            if (a.screen || a.fp32_tol > 0) {
+                const double dmax = quartet_dmax(a, mol, p, c.q);
+                if (a.screen) c.alive = qpq > a.cutoff && dmax * qpq > a.cutoff;
+                small = dmax * qpq < a.fp32_tol;
+            } else {
+                small = false;
+            }
+        }
+        c.fac = 0.0;
+        if (c.alive) {
+            c.pk0 = a.P.prim0[c.q];
+            c.npk = a.P.nprim[c.q];
+            c.CD = a.P.rAB[c.q];
+            double f = diagA ? 0.5 : 1.0;
+            if (a.P.shA[c.q] == a.P.shB[c.q]) f *= 0.5;
+            if (c.q == p && a.diag) f *= 0.5;
+            c.fac = f;
+        }
+        if (!__ballot(c.alive)) continue;   // wave-uniform
+        const bool use32 = a.fp32_tol > 0 && __ballot(c.alive && !small) == 0;
+        if (c.alive) {
+            cnt[use32 ? 2 : 0] += 1;
+            cnt[use32 ? 3 : 1] += (unsigned long long)(c.npb * c.npk);
+        }
+        if (use32) row_digest<LA, LB, LC, LD, SPLIT, S, float>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
+        else row_digest<LA, LB, LC, LD, SPLIT, S, double>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
+    }
+    __syncthreads();
+    // flush the accumulated rows of K' and the J'[a,b] block
+    for (int e = tid; e < a.n_k * krow; e += 256) {
+        const double v = kr[e];
+        if (v == 0.0) continue;
+        const int s = e / krow;
+        const int r = (e - s * krow) / ld;
+        const int col = e - s * krow - r * ld;
+        const int row = r < NA ? oa + r : ob + (r - NA);
+        atomic_add(a.K + cp * a.kcopy + s * a.slot + base + (long long)row * ld + col, v);
+    }
+    for (int e = tid; e < a.n_j * NA * NB; e += 256) {
+        const double v = jr[e];
+        if (v == 0.0) continue;
+        const int s = e / (NA * NB);
+        const int ia = (e / NB) % NA, ib2 = e % NB;
+        atomic_add(a.J + cp * a.jcopy + s * a.slot + base + (long long)(oa + ia) * ld + ob + ib2, v);
+    }
+    if (S == 0 && a.counters) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            unsigned long long v = cnt[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0 && v) atomicAdd(a.counters + k, v);
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD, int S>
+int launch_jk_row(int nblocks, const JKArgs& a, size_t shmem, hipStream_t st)
+{
+    constexpr int SPLIT = jk_split(LA, LB, LC, LD);
+    hipLaunchKernelGGL((k_jk_row<LA, LB, LC, LD, SPLIT, S>), dim3(nblocks), dim3(256), shmem, st, a);
+    return (int)hipGetLastError();
+}
+
 
 }  // namespace mjk
 

@@ -33,6 +33,7 @@
 
 namespace mjk {
 int launch_jk(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, hipStream_t st);
+int launch_jk_rows(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, size_t shmem, hipStream_t st);
 int launch_qcond(int la, int lb, const QCondArgs& a, hipStream_t st);
 int upload_c2s(hipStream_t st);
 int launch_sph2cart(const XformArgs& x, int maxld, const double* Dsph, long long sslot, int n_dm,
@@ -59,6 +60,7 @@ constexpr int NQCLASS_MAX = NPAIRCLASS * (NPAIRCLASS + 1) / 2;
 // Shell pairs of a class are grouped by their number of primitive pairs so
 // that every tile runs a uniform primitive loop (no lane divergence).
 constexpr int NBUCKET = 16;
+static_assert(NBUCKET <= ROW_MAXBUCKET, "row kernel holds at most ROW_MAXBUCKET ket buckets");
 int prim_bucket(int n)
 {
     static const int edge[NBUCKET - 1] = {1, 2, 3, 4, 6, 9, 12, 16, 20, 25, 32, 40, 50, 64, 100};
@@ -117,6 +119,7 @@ double flops_class(int a, int b, int c, int d, double nq, double nprimq)
 struct mjk_ctx {
     int device = 0;
     int ncopy = 8;        // private J'/K' accumulator copies (MI355X has 8 XCDs)
+    size_t row_lds = 61440;   // LDS bytes a row-kernel block may take; 0 = tile kernels only
     hipStream_t stream = nullptr;
     size_t budget = 0, used = 0;
     std::string err;
@@ -195,6 +198,8 @@ struct mjk_basis {
     int *prim0 = nullptr, *nprim = nullptr, *shA = nullptr, *shB = nullptr, *cartA = nullptr, *cartB = nullptr;
     double4 *rA = nullptr, *rAB = nullptr;
     std::vector<int> seg;            // [(class * nmol + mol) * NBUCKET + bucket] -> first pair
+    int* d_seg = nullptr;            // the same on the device
+    int* d_pmol = nullptr;           // molecule of every pair
     std::vector<double> q;           // Schwarz bound per pair (sorted order)
     std::vector<int> h_shA, h_shB, h_nprim;
     // per molecule / per AO device arrays
@@ -589,6 +594,15 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     b->q = qflat;
     flatten(b->seg);
     if ((rc = upload_pairs(&b->q))) return rc;
+    {
+        std::vector<int> pmol(std::max(np, 1), 0);
+        for (int X = 0; X < NPAIRCLASS; ++X)
+            for (int m = 0; m < nmol; ++m)
+                for (int p = b->seg[((size_t)X * nmol + m) * NBUCKET]; p < b->seg[((size_t)X * nmol + m + 1) * NBUCKET]; ++p)
+                    pmol[p] = m;
+        if ((rc = track_upload(b, b->seg, &b->d_seg))) return rc;
+        if ((rc = track_upload(b, pmol, &b->d_pmol))) return rc;
+    }
 
     // per molecule / AO maps
     std::vector<long long> dbase, cbase, sbase, cfirst, sfirst, hfirst;
@@ -680,6 +694,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     (void)hipMemGetInfo(&fr, &tot);
     ctx->budget = hbm_budget ? hbm_budget : (size_t)(0.8 * (double)fr);
     if (const char* nc = getenv("MJK_NCOPY")) ctx->ncopy = std::max(1, std::min(16, atoi(nc)));
+    if (const char* rl = getenv("MJK_ROW_LDS")) ctx->row_lds = (size_t)std::max(0, std::min(61440, atoi(rl)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     std::vector<double> tab(RYS_BUF_SIZE);
     rys_fill_buffer(tab.data());
@@ -712,6 +727,15 @@ const char* mjk_last_error(const mjk_ctx* ctx)
 size_t mjk_ctx_hbm_used(const mjk_ctx* ctx)
 {
     return ctx ? ctx->used : 0;
+}
+
+int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes)
+{
+    if (!ctx) return MJK_EINVAL;
+    if (bytes > 61440) return fail(ctx, MJK_EINVAL, "row kernel LDS limit is 61440 bytes");
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    ctx->row_lds = bytes;
+    return MJK_OK;
 }
 
 int mjk_basis_upload_batch(mjk_ctx* ctx, int nmol, const int* natm, const int* nbas, const int* const* atm,
@@ -849,6 +873,8 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.screen = screen ? 1 : 0;
     a.fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
     a.rys = ctx->rys;
+    a.pmol = b->d_pmol;
+    a.nbucket = NBUCKET;
     const bool prof = b->profiling && b->pev.size() >= 2 * b->qcls.size();
     for (size_t c = 0; c < b->qcls.size(); ++c) {
         const QClass& qc = b->qcls[c];
@@ -860,7 +886,24 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         a.nbt = qc.nbt;
         a.diag = qc.X == qc.Y;
         a.counters = b->counters + 4 * c;
-        if (launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
+        // row kernel (one block per bra pair, K' rows in LDS) when the rows fit
+        const size_t shmem = sizeof(double) * ((size_t)n_k * (ncart(qc.la) + ncart(qc.lb)) * b->maxld +
+                                               (size_t)n_j * ncart(qc.la) * ncart(qc.lb));
+        const int pb = b->seg[(size_t)qc.X * b->nmol * NBUCKET];
+        const int nbra = b->seg[(size_t)(qc.X + 1) * b->nmol * NBUCKET] - pb;
+        int rr = -1;   // -1: no row kernel for this class (cooperative classes)
+        if (shmem <= ctx->row_lds && nbra > 0) {
+            // enough blocks to fill the chip, at least ~128 ket candidates per block
+            const double kets = (double)qc.nblocks * TILE * TILE / nbra;
+            const int want = (3072 + nbra - 1) / nbra;
+            a.rsplit = std::max(1, std::min(want, (int)(kets / 128)));
+            a.pbase = pb;
+            a.nbra = nbra;
+            a.kseg = b->d_seg + (size_t)qc.Y * b->nmol * NBUCKET;
+            rr = launch_jk_rows(qc.la, qc.lb, qc.lc, qc.ld, nbra * a.rsplit, a, shmem, st);
+            if (rr > 0) return fail(ctx, MJK_EDEVICE, "row kernel launch failed for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
+        }
+        if (rr != 0 && launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
             return fail(ctx, MJK_EUNSUPPORTED, "no kernel for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
         if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c + 1], st));
     }

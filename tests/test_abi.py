@@ -36,6 +36,7 @@ def test_bad_arguments_return_codes():
     assert l.mjk_rys_roots(3, -1.0, p(x), p(w)) == -1
     assert l.mjk_ctx_create(None, 0, 0) == -1
     assert l.mjk_last_error(None) == b'no context'
+    assert l.mjk_ctx_set_row_lds(None, 0) == -1
 
 
 @pytest.mark.parametrize('n', range(1, 10))

@@ -153,3 +153,28 @@ def test_cooperative_classes_dd(oracle):
     vj, vk = hf.get_jk(mol, dm, vhfopt=opt)
     j0, k0 = oracle.get_jk(mol, dm, vhfopt=oracle.VHFOpt(mol, 1e-13))
     assert abs(vj - j0).max() < TOL and abs(vk - k0).max() < TOL
+
+
+@pytest.mark.parametrize('row_lds', [0, 3072])
+def test_row_and_tile_kernels_agree(oracle, row_lds):
+    # row kernels (K' rows in LDS) against tile kernels: 0 forces tile kernels
+    # everywhere, 3072 bytes keeps row kernels only for the narrowest classes
+    mol = gto.M(atom='''O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587
+                        O 3.1 0.2 0; H 3.1 -0.557 0.587; H 3.1 0.957 0.587''', basis='def2-svp')
+    nao = mol.nao_nr()
+    rng = numpy.random.default_rng(13)
+    c = rng.standard_normal((nao, 10))
+    dms = numpy.array([c @ c.T / 10, rng.standard_normal((nao, nao))])
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    eng = _vhf.get_engine()
+    ref = [hf.get_jk(mol, dms[0], hermi=1, vhfopt=opt), hf.get_jk(mol, dms, hermi=0, vhfopt=opt)]
+    assert eng.lib.mjk_ctx_set_row_lds(eng.handle, row_lds) == 0
+    try:
+        got = [hf.get_jk(mol, dms[0], hermi=1, vhfopt=opt), hf.get_jk(mol, dms, hermi=0, vhfopt=opt)]
+    finally:
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 61440) == 0
+    for (j0, k0), (j1, k1) in zip(ref, got):
+        assert abs(j1 - j0).max() < 1e-11 * max(1, abs(j0).max())
+        assert abs(k1 - k0).max() < 1e-11 * max(1, abs(k0).max())
+    j0, k0 = oracle.get_jk(mol, dms, hermi=0, vhfopt=oracle.VHFOpt(mol, 1e-13))
+    assert abs(got[1][0] - j0).max() < TOL * 10 and abs(got[1][1] - k0).max() < TOL * 10
