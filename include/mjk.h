@@ -123,11 +123,13 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n
                       double* d_vj, double* d_vk, mjk_stats* stats);
 
 /* Per-quartet-class kernel profiling with HIP events on the launch stream.
- * mjk_profile(…, 1) resets and starts accumulating (each J/K build then
- * synchronises at its end); mjk_profile_read returns the number of classes
- * written: cls[4k..4k+3] = (la lb lc ld), accumulated GPU ms, evaluated
- * quartets, primitive quartets, algorithmic FLOPs and launches.  No
- * reference counterpart (the reference's timer is logger.timer,
+ * mjk_profile(…, 1) resets and starts accumulating times and work tallies
+ * (each J/K build then synchronises at its end); mjk_profile(…, 2) does the
+ * same with times only (the kernels then keep no tallies, as in an
+ * unprofiled build); 0 stops.  mjk_profile_read returns the number of
+ * classes written: cls[4k..4k+3] = (la lb lc ld), accumulated GPU ms,
+ * evaluated quartets, primitive quartets, algorithmic FLOPs and launches.
+ * No reference counterpart (the reference's timer is logger.timer,
  * pyscf/lib/logger.py:167-180). */
 int mjk_profile(mjk_ctx* ctx, mjk_basis* basis, int enable);
 int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* basis, int max_classes, int* cls, double* ms,

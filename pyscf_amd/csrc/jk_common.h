@@ -74,11 +74,12 @@ struct JKArgs {
     double fp32_tol;        // > 0: waves whose quartets all have q_ij q_kl dmax < fp32_tol run in FP32
     const double* rys;      // Rys table buffer
     unsigned long long* counters;  // {FP64 quartets, FP64 primitive quartets, FP32 quartets, FP32 primitive quartets}
-    // row kernel: one block per bra pair p = pbase + blockIdx.x against all kets of the ket class
-    int pbase, nbra;        // bra pairs pbase .. pbase + nbra - 1
-    int rsplit;             // blocks sharing the kets of one bra pair
+    // row kernel: block b takes bra pair rt[b].x and the kets rt[b].y .. min(rt[b].z, n) - 1 of
+    // the n surviving kets of the ket class, i.e. of the concatenated ket ranges
+    // rr[rt[b].w * nbucket + k] = {first, end} (one per primitive bucket k)
+    const int4* rt;
+    const int2* rr;
     const int* pmol;        // molecule of every pair
-    const int* kseg;        // ket class segment starts: [mol * nbucket + bucket], one past = end
     int nbucket;
 };
 

@@ -481,7 +481,7 @@ __global__ __launch_bounds__(64) void k_jk(const JKArgs a)
     if (!__ballot(c.alive)) return;   // whole tile screened out (wave-uniform exit)
     // FP32 only when every live quartet of the wave is below the bound
     const bool use32 = a.fp32_tol > 0 && __ballot(c.alive && !small) == 0;
-    if (S == 0) count_work(a.counters + (use32 ? 2 : 0), c.alive, c.npb * c.npk);
+    if (S == 0 && a.counters) count_work(a.counters + (use32 ? 2 : 0), c.alive, c.npb * c.npk);
     if (use32) quartet_slice<LA, LB, LC, LD, SPLIT, S, float>(a, c);
     else quartet_slice<LA, LB, LC, LD, SPLIT, S, double>(a, c);
 }
@@ -609,6 +609,7 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
         oz[c] = 2 * ISZ + ((za * (LB + 1) + zb) * (LC + 1) + zc) * (LD + 1) + zd;
     }
 
+    unsigned long long nq_blk = 0, np_blk = 0;
     for (int col = 0; col < TILE; ++col) {
         const int q = t.z + kt * TILE + col;
         if (q >= t.w || (a.diag && q > p)) break;
@@ -618,10 +619,8 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
             if (!(qpq > a.cutoff && quartet_dmax(a, mol, p, q) * qpq > a.cutoff)) continue;   // block-uniform
         }
         const int pk0 = a.P.prim0[q], npk = a.P.nprim[q];
-        if (tid == 0 && a.counters) {
-            atomicAdd(a.counters, 1ULL);
-            atomicAdd(a.counters + 1, (unsigned long long)(npb * npk));
-        }
+        nq_blk += 1;
+        np_blk += (unsigned long long)(npb * npk);
         const double4 CD = a.P.rAB[q];
         double gout[CH];
 #pragma unroll
@@ -751,6 +750,10 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
 #undef BLK
         __syncthreads();   // the block region is rewritten by the next quartet
     }
+    if (tid == 0 && a.counters && nq_blk) {
+        atomicAdd(a.counters, nq_blk);
+        atomicAdd(a.counters + 1, np_blk);
+    }
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -783,10 +786,11 @@ int launch_jk_slice(int nblocks, const JKArgs& a, hipStream_t st)
 // ---------------------------------------------------------------------------
 // Row kernel.  Every K' contribution of a quartet (ab|cd) lands in rows a
 // and b of K' (K'[a,d], K'[a,c], K'[b,d], K'[b,c]).  A block of 256 threads
-// owns one bra pair and a share (1/rsplit) of the surviving kets of the ket
-// class: the Schwarz prefix of every primitive bucket is found by binary
-// search, the prefixes are concatenated, and each wave takes 64 consecutive
-// kets at a time.  The block accumulates its K' rows and its J'[a,b] block
+// owns one bra pair and one chunk of the surviving kets of the ket class:
+// the Schwarz prefix of every primitive bucket is found by binary search,
+// the prefixes are concatenated, and the block's waves claim 64 consecutive
+// kets of its chunk at a time from a shared counter.  Chunks of equal size keep the blocks'
+// run times even whatever the bra pair's ket count.  The block accumulates its K' rows and its J'[a,b] block
 // in LDS with LDS atomics and flushes the non-zero entries to HBM once; only
 // J'[c,d] is added per quartet.  LDS holds n_k (NA + NB) ld + n_j NA NB
 // doubles (dynamic shared memory).
@@ -939,10 +943,10 @@ void k_jk_row(const JKArgs a)
     constexpr int NROW = NA + NB;
     extern __shared__ double lds_rows[];
     __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // block b: bra pair pbase + b % nbra, ket chunks part, part + rsplit, ... of it
-    const int p = a.pbase + (int)(blockIdx.x % (unsigned)a.nbra);
-    const int part = (int)(blockIdx.x / (unsigned)a.nbra);
+    __shared__ int s_next;   // next unclaimed ket of the chunk (waves take 64 at a time)
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int4 task = a.rt[blockIdx.x];
+    const int p = task.x;
     const int mol = a.pmol[p];
     const int ld = a.M.ld[mol];
     const long long base = a.M.dbase[mol];
@@ -952,30 +956,21 @@ void k_jk_row(const JKArgs a)
     const int nz = a.n_k * krow + a.n_j * NA * NB;
     for (int e = tid; e < nz; e += 256) lds_rows[e] = 0.0;
     const double qp = a.P.rA[p].w;
-    // surviving ket range of every primitive bucket: Schwarz prefix of the
-    // (descending) segment, and q <= p on the diagonal class
+    // surviving ket range of every primitive bucket (Schwarz prefix of the
+    // descending segment, q <= p on the diagonal class), from the task list
     if (tid < a.nbucket) {
-        const int ys = a.kseg[mol * a.nbucket + tid], ye = a.kseg[mol * a.nbucket + tid + 1];
-        int yend = ye;
-        if (a.screen) {
-            int lo = ys, hi = ye;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (a.P.rA[mid].w * qp > a.cutoff) lo = mid + 1; else hi = mid;
-            }
-            yend = lo;
-        }
-        if (a.diag && yend > p + 1) yend = p + 1;
-        s_start[tid] = ys;
-        s_pre[tid + 1] = yend > ys ? yend - ys : 0;
+        const int2 r = a.rr[(long long)task.w * a.nbucket + tid];
+        s_start[tid] = r.x;
+        s_pre[tid + 1] = r.y - r.x;
     }
     __syncthreads();
     if (tid == 0) {
         s_pre[0] = 0;
         for (int k = 1; k <= a.nbucket; ++k) s_pre[k] += s_pre[k - 1];
+        s_next = task.y;
     }
     __syncthreads();
-    const int total = s_pre[a.nbucket];
+    const int iend = min(s_pre[a.nbucket], task.z);
 
     QuartetCtx c;
     c.p = p;
@@ -989,9 +984,13 @@ void k_jk_row(const JKArgs a)
     const int oa = a.P.cartA[p], ob = a.P.cartB[p];
     const int cp = xcc_copy(a.ncopy);
     unsigned long long cnt[4] = {0, 0, 0, 0};
-    for (int i0 = (part * 4 + wave) * 64; i0 < total; i0 += 256 * a.rsplit) {
+    for (;;) {
+        int i0 = 0;
+        if (lane == 0) i0 = atomicAdd(&s_next, 64);
+        i0 = __shfl(i0, 0);
+        if (i0 >= iend) break;   // wave-uniform
         const int idx = i0 + lane;
-        c.qvalid = idx < total;
+        c.qvalid = idx < iend;
         c.q = 0;
         if (c.qvalid) {
             int lo = 0, hi = a.nbucket;   // bucket k with s_pre[k] <= idx < s_pre[k + 1]
@@ -1053,14 +1052,19 @@ void k_jk_row(const JKArgs a)
         const int ia = (e / NB) % NA, ib2 = e % NB;
         atomic_add(a.J + cp * a.jcopy + s * a.slot + base + (long long)(oa + ia) * ld + ob + ib2, v);
     }
-    if (S == 0 && a.counters) {
+    if (S == 0 && a.counters) {   // one atomic per counter and block
+        __shared__ unsigned long long s_cnt[4];
+        if (tid < 4) s_cnt[tid] = 0;
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             unsigned long long v = cnt[k];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0 && v) atomicAdd(a.counters + k, v);
+            if (lane == 0 && v) atomicAdd(s_cnt + k, v);
         }
+        __syncthreads();
+        if (tid < 4 && s_cnt[tid]) atomicAdd(a.counters + tid, s_cnt[tid]);
     }
 }
 

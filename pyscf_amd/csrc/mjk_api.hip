@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -34,6 +35,7 @@
 namespace mjk {
 int launch_jk(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, hipStream_t st);
 int launch_jk_rows(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, size_t shmem, hipStream_t st);
+int row_kernel_available(int la, int lb, int lc, int ld);
 int launch_qcond(int la, int lb, const QCondArgs& a, hipStream_t st);
 int upload_c2s(hipStream_t st);
 int launch_sph2cart(const XformArgs& x, int maxld, const double* Dsph, long long sslot, int n_dm,
@@ -90,7 +92,13 @@ struct QClass {
     int4* bt = nullptr;
     int* bt_mol = nullptr;
     int* bt_off = nullptr;
+    // row-kernel tasks {bra pair, first ket, end of chunk or INT_MAX for the last chunk, range set}
+    // and per-bucket ket ranges {first, end} of each bra pair (range set r = rr[r * NBUCKET ...])
+    int nrt = 0;
+    int4* rt = nullptr;
+    int2* rr = nullptr;
 };
+
 
 int lmax_built()
 {
@@ -120,6 +128,9 @@ struct mjk_ctx {
     int device = 0;
     int ncopy = 8;        // private J'/K' accumulator copies (MI355X has 8 XCDs)
     size_t row_lds = 61440;   // LDS bytes a row-kernel block may take; 0 = tile kernels only
+    int row_chunk = 16384;    // least primitive quartets per row-kernel block
+    int row_blocks = 16384;   // row-kernel blocks per class launch to aim for ...
+    int row_min_blocks = 512;    // ... and the fewest, whatever row_chunk asks
     hipStream_t stream = nullptr;
     size_t budget = 0, used = 0;
     std::string err;
@@ -198,10 +209,10 @@ struct mjk_basis {
     int *prim0 = nullptr, *nprim = nullptr, *shA = nullptr, *shB = nullptr, *cartA = nullptr, *cartB = nullptr;
     double4 *rA = nullptr, *rAB = nullptr;
     std::vector<int> seg;            // [(class * nmol + mol) * NBUCKET + bucket] -> first pair
-    int* d_seg = nullptr;            // the same on the device
     int* d_pmol = nullptr;           // molecule of every pair
     std::vector<double> q;           // Schwarz bound per pair (sorted order)
     std::vector<int> h_shA, h_shB, h_nprim;
+    std::vector<long long> pcost;    // prefix sums of h_nprim (row-kernel task sizes)
     // per molecule / per AO device arrays
     long long *d_dbase = nullptr, *d_cbase = nullptr, *d_sbase = nullptr, *d_cart_first = nullptr,
               *d_sph_first = nullptr, *d_sh_first = nullptr;
@@ -220,6 +231,7 @@ struct mjk_basis {
     std::vector<void*> owned;
     // per-class profiling (HIP events around every class launch)
     bool profiling = false;
+    bool prof_counts = false;             // tallies (quartets, primitive quartets) while profiling
     std::vector<hipEvent_t> pev;          // 2 per quartet class
     std::vector<double> prof_ms;
     std::vector<long long> prof_nq, prof_np, prof_launches;
@@ -233,6 +245,8 @@ void free_tiles(mjk_basis* b)
         dfree(b->ctx, c.bt);
         dfree(b->ctx, c.bt_mol);
         dfree(b->ctx, c.bt_off);
+        dfree(b->ctx, c.rt);
+        dfree(b->ctx, c.rr);
     }
     b->qcls.clear();
     b->tile_cutoff = -1.0;
@@ -300,6 +314,90 @@ int build_tiles(mjk_basis* b, double cutoff)
                 if ((rc = upload(ctx, bt, &qc.bt))) return rc;
                 if ((rc = upload(ctx, btm, &qc.bt_mol))) return rc;
                 if ((rc = upload(ctx, off, &qc.bt_off))) return rc;
+            }
+            // Row-kernel tasks: the same Schwarz prefixes per bra pair, cut into
+            // chunks of about equal primitive-quartet count (the kets' primitive
+            // counts differ up to a hundredfold).
+            if (qc.nbt > 0 && row_kernel_available(qc.la, qc.lb, qc.lc, qc.ld)) {
+                std::vector<int2> rr;
+                std::vector<int> bra;
+                long long cost_all = 0;
+                for (int m = 0; m < nmol; ++m) {
+                    const size_t sx = ((size_t)X * nmol + m) * NBUCKET, sy = ((size_t)Y * nmol + m) * NBUCKET;
+                    for (int p = b->seg[sx]; p < b->seg[sx + NBUCKET]; ++p) {
+                        long long np_p = 0;
+                        for (int by = 0; by < NBUCKET; ++by) {
+                            const int y0 = b->seg[sy + by], y1 = b->seg[sy + by + 1];
+                            int yend = y1;
+                            if (cutoff > 0) {
+                                int lo = y0, hi = y1;
+                                while (lo < hi) {
+                                    const int mid = (lo + hi) / 2;
+                                    if (b->q[mid] * b->q[p] > cutoff) lo = mid + 1; else hi = mid;
+                                }
+                                yend = lo;
+                            }
+                            if (X == Y) yend = std::min(yend, p + 1);
+                            yend = std::max(yend, y0);
+                            rr.push_back(make_int2(y0, yend));
+                            np_p += b->pcost[yend] - b->pcost[y0];
+                        }
+                        if (np_p == 0) {
+                            rr.resize(rr.size() - NBUCKET);
+                            continue;
+                        }
+                        bra.push_back(p);
+                        cost_all += (long long)b->h_nprim[p] * np_p;
+                    }
+                }
+                // enough blocks to fill the chip, each long enough to amortise its
+                // fixed cost (LDS clear, range set-up, row flush)
+                const long long budget = std::min(std::max(cost_all / ctx->row_blocks, (long long)ctx->row_chunk),
+                                                  std::max(cost_all / ctx->row_min_blocks, 1LL));
+                std::vector<std::pair<long long, int4>> tasks;
+                for (size_t ib = 0; ib < bra.size(); ++ib) {
+                    const int p = bra[ib];
+                    const long long npb = b->h_nprim[p];
+                    int flat = 0, start = 0;
+                    long long cur = 0;
+                    for (int k = 0; k < NBUCKET; ++k) {
+                        int y = rr[ib * NBUCKET + k].x;
+                        const int y1 = rr[ib * NBUCKET + k].y;
+                        while (y < y1) {
+                            const long long rest = npb * (b->pcost[y1] - b->pcost[y]);
+                            if (cur + rest < budget) {
+                                cur += rest;
+                                flat += y1 - y;
+                                break;
+                            }
+                            // smallest y' with npb * (pcost[y'] - pcost[y]) >= budget - cur
+                            int lo = y + 1, hi = y1;
+                            while (lo < hi) {
+                                const int mid = (lo + hi) / 2;
+                                if (npb * (b->pcost[mid] - b->pcost[y]) >= budget - cur) hi = mid; else lo = mid + 1;
+                            }
+                            cur += npb * (b->pcost[lo] - b->pcost[y]);
+                            flat += lo - y;
+                            y = lo;
+                            tasks.push_back({cur, make_int4(p, start, flat, (int)ib)});
+                            start = flat;
+                            cur = 0;
+                        }
+                    }
+                    if (flat > start) tasks.push_back({cur, make_int4(p, start, flat, (int)ib)});
+                    tasks.back().second.z = INT_MAX;   // the bra pair's last chunk runs to its end
+                }
+                // heaviest tasks first (the last chunk of a bra pair is the short one)
+                std::stable_sort(tasks.begin(), tasks.end(),
+                                 [](const std::pair<long long, int4>& u, const std::pair<long long, int4>& v) {
+                                     return u.first > v.first;
+                                 });
+                if (tasks.size() > 2000000000ULL) return fail(ctx, MJK_EUNSUPPORTED, "too many row-kernel tasks");
+                std::vector<int4> rt(tasks.size());
+                for (size_t t = 0; t < tasks.size(); ++t) rt[t] = tasks[t].second;
+                qc.nrt = (int)rt.size();
+                int rc;
+                if (qc.nrt > 0 && ((rc = upload(ctx, rt, &qc.rt)) || (rc = upload(ctx, rr, &qc.rr)))) return rc;
             }
             b->qcls.push_back(qc);
         }
@@ -594,13 +692,14 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     b->q = qflat;
     flatten(b->seg);
     if ((rc = upload_pairs(&b->q))) return rc;
+    b->pcost.assign(np + 1, 0);
+    for (int p = 0; p < np; ++p) b->pcost[p + 1] = b->pcost[p] + b->h_nprim[p];
     {
         std::vector<int> pmol(std::max(np, 1), 0);
         for (int X = 0; X < NPAIRCLASS; ++X)
             for (int m = 0; m < nmol; ++m)
                 for (int p = b->seg[((size_t)X * nmol + m) * NBUCKET]; p < b->seg[((size_t)X * nmol + m + 1) * NBUCKET]; ++p)
                     pmol[p] = m;
-        if ((rc = track_upload(b, b->seg, &b->d_seg))) return rc;
         if ((rc = track_upload(b, pmol, &b->d_pmol))) return rc;
     }
 
@@ -694,6 +793,9 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     (void)hipMemGetInfo(&fr, &tot);
     ctx->budget = hbm_budget ? hbm_budget : (size_t)(0.8 * (double)fr);
     if (const char* nc = getenv("MJK_NCOPY")) ctx->ncopy = std::max(1, std::min(16, atoi(nc)));
+    if (const char* rc = getenv("MJK_ROW_CHUNK")) ctx->row_chunk = std::max(256, std::min(1 << 24, atoi(rc)));
+    if (const char* rm = getenv("MJK_ROW_MIN_BLOCKS")) ctx->row_min_blocks = std::max(1, std::min(1 << 20, atoi(rm)));
+    if (const char* rb = getenv("MJK_ROW_BLOCKS")) ctx->row_blocks = std::max(256, std::min(1 << 20, atoi(rb)));
     if (const char* rl = getenv("MJK_ROW_LDS")) ctx->row_lds = (size_t)std::max(0, std::min(61440, atoi(rl)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     std::vector<double> tab(RYS_BUF_SIZE);
@@ -845,7 +947,9 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
     if (n_j) HIPCHK(ctx, hipMemsetAsync(b->J, 0, sizeof(double) * ncopy * jcopy, st));
     if (n_k) HIPCHK(ctx, hipMemsetAsync(b->K, 0, sizeof(double) * ncopy * kcopy, st));
-    HIPCHK(ctx, hipMemsetAsync(b->counters, 0, sizeof(unsigned long long) * 4 * NQCLASS_MAX, st));
+    const bool prof = b->profiling && b->pev.size() >= 2 * b->qcls.size();
+    const bool tally = stats || (prof && b->prof_counts);
+    if (tally) HIPCHK(ctx, hipMemsetAsync(b->counters, 0, sizeof(unsigned long long) * 4 * NQCLASS_MAX, st));
     if (need_cond && launch_dmcond(dmcond_args(b), d_dms, b->slot_s, n_dm, b->dmcond, st))
         return fail(ctx, MJK_EDEVICE, "dm_cond launch failed");
     if (launch_sph2cart(xform(b), b->maxld, d_dms, b->slot_s, n_dm, hermi, with_j, with_k, b->Dj, b->Dk,
@@ -875,7 +979,6 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.rys = ctx->rys;
     a.pmol = b->d_pmol;
     a.nbucket = NBUCKET;
-    const bool prof = b->profiling && b->pev.size() >= 2 * b->qcls.size();
     for (size_t c = 0; c < b->qcls.size(); ++c) {
         const QClass& qc = b->qcls[c];
         if (qc.nblocks == 0) continue;
@@ -885,22 +988,15 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         a.bt_off = qc.bt_off;
         a.nbt = qc.nbt;
         a.diag = qc.X == qc.Y;
-        a.counters = b->counters + 4 * c;
+        a.counters = tally ? b->counters + 4 * c : nullptr;   // work tallies only on request
         // row kernel (one block per bra pair, K' rows in LDS) when the rows fit
         const size_t shmem = sizeof(double) * ((size_t)n_k * (ncart(qc.la) + ncart(qc.lb)) * b->maxld +
                                                (size_t)n_j * ncart(qc.la) * ncart(qc.lb));
-        const int pb = b->seg[(size_t)qc.X * b->nmol * NBUCKET];
-        const int nbra = b->seg[(size_t)(qc.X + 1) * b->nmol * NBUCKET] - pb;
         int rr = -1;   // -1: no row kernel for this class (cooperative classes)
-        if (shmem <= ctx->row_lds && nbra > 0) {
-            // enough blocks to fill the chip, at least ~128 ket candidates per block
-            const double kets = (double)qc.nblocks * TILE * TILE / nbra;
-            const int want = (3072 + nbra - 1) / nbra;
-            a.rsplit = std::max(1, std::min(want, (int)(kets / 128)));
-            a.pbase = pb;
-            a.nbra = nbra;
-            a.kseg = b->d_seg + (size_t)qc.Y * b->nmol * NBUCKET;
-            rr = launch_jk_rows(qc.la, qc.lb, qc.lc, qc.ld, nbra * a.rsplit, a, shmem, st);
+        if (shmem <= ctx->row_lds && qc.nrt > 0) {
+            a.rt = qc.rt;
+            a.rr = qc.rr;
+            rr = launch_jk_rows(qc.la, qc.lb, qc.lc, qc.ld, qc.nrt, a, shmem, st);
             if (rr > 0) return fail(ctx, MJK_EDEVICE, "row kernel launch failed for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
         }
         if (rr != 0 && launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
@@ -914,9 +1010,10 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
     if (prof) {
         HIPCHK(ctx, hipStreamSynchronize(st));
-        std::vector<unsigned long long> cnt(4 * NQCLASS_MAX);
-        HIPCHK(ctx, hipMemcpy(cnt.data(), b->counters, sizeof(unsigned long long) * cnt.size(),
-                              hipMemcpyDeviceToHost));
+        std::vector<unsigned long long> cnt(4 * NQCLASS_MAX, 0);
+        if (tally)
+            HIPCHK(ctx, hipMemcpy(cnt.data(), b->counters, sizeof(unsigned long long) * cnt.size(),
+                                  hipMemcpyDeviceToHost));
         for (size_t c = 0; c < b->qcls.size(); ++c) {
             if (b->qcls[c].nblocks == 0) continue;
             float ms = 0;
@@ -1001,7 +1098,9 @@ int mjk_profile(mjk_ctx* ctx, mjk_basis* b, int enable)
     b->prof_nq.assign(n, 0);
     b->prof_np.assign(n, 0);
     b->prof_launches.assign(n, 0);
+    if (enable < 0 || enable > 2) return fail(ctx, MJK_EINVAL, "profile mode must be 0, 1 or 2");
     b->profiling = enable != 0;
+    b->prof_counts = enable == 1;
     return MJK_OK;
 }
 
