@@ -926,13 +926,16 @@ __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c,
 }
 
 // Minimum waves per SIMD the compiler must allow for the row kernel of a
-// class (caps its VGPR budget at 512 / value).
+// class (caps its VGPR budget at 512 / value).  Two for the (dp|p s) and
+// (ds|pp) classes, whose unconstrained allocation leaves one wave per SIMD
+// (measured on MI355X: 10-15% faster); the others run best unconstrained.
 #ifndef MJK_ROW_WPE
 #define MJK_ROW_WPE 1
 #endif
 template <int LA, int LB, int LC, int LD>
 struct RowOccupancy {
-    static constexpr int value = MJK_ROW_WPE;
+    static constexpr int value =
+        (LA == 2 && LB == 1 && LC == 1 && LD == 0) || (LA == 2 && LB == 0 && LC == 1 && LD == 1) ? 2 : MJK_ROW_WPE;
 };
 
 template <int LA, int LB, int LC, int LD, int SPLIT, int S>
