@@ -32,6 +32,31 @@ METRIC = 'JK builds/s + ERI quartets/s, mixed vs FP64, at <=1e-8 Eh; p50/p99 lat
 FP64_PEAK = 78.6e12
 
 
+def kernel_name(cls, kind):
+    """Demangled name (as rocprofv3 prints it, less the argument list) of the
+    kernel a quartet class ran on: 0 tile, 1 row, 2 cooperative; classes cut
+    into slices run one kernel per slice and are named by slice 0."""
+    from pyscf_amd.build import jk_split
+    la, lb, lc, ld = cls
+    if kind == 2:
+        return 'mjk::k_jk_coop<%d, %d, %d, %d>' % cls
+    base = 'mjk::k_jk_row' if kind == 1 else 'mjk::k_jk'
+    return '%s<%d, %d, %d, %d, %d, 0>' % (base, la, lb, lc, ld, jk_split(la, lb, lc, ld))
+
+
+def traffic_bytes(kernel):
+    """HBM bytes per launch of `kernel` on the c2 workload, from the calibrated
+    FETCH_SIZE and WRITE_SIZE counter runs summarised in profiles/ (None if
+    the kernel has no entry there)."""
+    path = os.path.join(ROOT, 'profiles', 'jk_c2_traffic.json')
+    try:
+        with open(path) as f:
+            rec = json.load(f)['kernels'][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return rec['fetch_bytes_per_launch'] + rec['write_bytes_per_launch']
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -43,6 +68,10 @@ def parse():
     ap.add_argument('--rate', type=float, default=20.0, help='c5: Poisson arrival rate (1/s)')
     ap.add_argument('--max-ao', type=int, default=1500, help='c5: largest request (AOs)')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--dm-out', default=None, help='c2: save the converged density (.npy)')
+    ap.add_argument('--dm-in', default=None,
+                    help='c2: load the density from a .npy written by --dm-out instead of running the SCF '
+                         '(profiling runs: every J/K launch is then one of the converged density)')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--fp32-tol', type=float, default=1e-7)
     return ap.parse_args()
@@ -197,15 +226,25 @@ def main():
     nao = mol.nao_nr()
     tol = 1e-13
     t0 = time.perf_counter()
-    mf = rhf.RHF(mol, direct_scf_tol=tol)
-    mf.conv_tol = 1e-10
-    e64 = mf.kernel()
-    t_scf = time.perf_counter() - t0
-    dm = numpy.ascontiguousarray(mf.dm)
+    if args.dm_in:
+        dm = numpy.ascontiguousarray(numpy.load(args.dm_in, allow_pickle=False))
+        opt = _vhf.VHFOpt(mol, direct_scf_tol=tol)
+        e64 = None
+        scf_info = {'skipped': 'density from %s' % os.path.basename(args.dm_in)}
+    else:
+        mf = rhf.RHF(mol, direct_scf_tol=tol)
+        mf.conv_tol = 1e-10
+        e64 = mf.kernel()
+        dm = numpy.ascontiguousarray(mf.dm)
+        opt = mf._opt
+        scf_info = {'e_tot': e64, 'cycles': mf.cycles, 'converged': mf.converged,
+                    'seconds': time.perf_counter() - t0}
+        if args.dm_out:
+            numpy.save(args.dm_out, dm)
 
-    eng = mf._opt.engine
+    eng = opt.engine
     L = eng.lib
-    basis = mf._opt.basis
+    basis = opt.basis
     ctx = eng.handle
     nbytes = dm.nbytes
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
@@ -250,13 +289,16 @@ def main():
         eng.check(L.mjk_synchronize(ctx), 'sync')
         t_total = time.perf_counter() - t_start
         k = L.mjk_profile_read(ctx, basis.handle, maxc, ptr(cls), ptr(ms), ptr(nq), ptr(npq), ptr(fl), ptr(nl))
+        kinds = numpy.zeros(maxc, dtype=numpy.int32)
+        L.mjk_profile_kinds(ctx, basis.handle, maxc, ptr(kinds))
         L.mjk_profile(ctx, basis.handle, 0)
         classes = []
         for i in range(k):
-            key = tuple(cls[4 * i:4 * i + 4])
+            key = tuple(int(x) for x in cls[4 * i:4 * i + 4])
             nq1, fl1 = work.get(key, (0, 0.0))
             t1 = ms[i] / nl[i]
-            classes.append({'class': '(%d%d|%d%d)' % key, 'ms_per_launch': t1, 'quartets': nq1, 'flops': fl1,
+            classes.append({'class': '(%d%d|%d%d)' % key, 'kernel': kernel_name(key, int(kinds[i])),
+                            'ms_per_launch': t1, 'quartets': nq1, 'flops': fl1,
                             'tflops': fl1 / (t1 * 1e-3) / 1e12 if t1 > 0 else 0.0})
         classes.sort(key=lambda c: -c['ms_per_launch'])
         vj = numpy.empty_like(dm)
@@ -271,7 +313,7 @@ def main():
         L.mjk_device_free(ctx, p)
     # mixed-precision SCF energy
     emx = None
-    if rmx is not None:
+    if rmx is not None and e64 is not None:
         mfm = rhf.RHF(mol, direct_scf_tol=tol, fp32_tol=args.fp32_tol)
         mfm.conv_tol = 1e-10
         emx = mfm.kernel()
@@ -300,12 +342,13 @@ def main():
         'quartets_per_build': int(st.quartets),
         'p50_ms': float(numpy.percentile(lat, 50) * 1e3),
         'p99_ms': float(numpy.percentile(lat, 99) * 1e3),
-        'scf': {'e_tot': e64, 'cycles': mf.cycles, 'converged': mf.converged, 'seconds': t_scf},
+        'scf': scf_info,
         'gpu_kernel_ms_per_build': kernel_ms,
         'fp64_tflops_all_classes': total_flops / (kernel_ms * 1e-3) / 1e12,
-        'roofline': {'bound': 'fp64', 'kernel': 'k_jk' + dom['class'],
+        'roofline': {'bound': 'fp64', 'kernel': dom['kernel'], 'class': dom['class'],
                      'achieved': dom['tflops'], 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
-                     'frac': dom['tflops'] * 1e12 / FP64_PEAK, 'traffic': None,
+                     'frac': dom['tflops'] * 1e12 / FP64_PEAK,
+                     'traffic': traffic_bytes(dom['kernel']) if args.workload == 'c2' else None,
                      'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch']},
         'classes': classes[:8],
     }
@@ -322,7 +365,7 @@ def main():
             'max_abs_dJ': float(abs(rmx['vj'] - r64['vj']).max()),
             'max_abs_dK': float(abs(rmx['vk'] - r64['vk']).max()),
             'e_tot': emx,
-            'dE_vs_fp64': emx - e64,
+            'dE_vs_fp64': emx - e64 if emx is not None else None,
         }
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(mol, dm, tol, args.cpu_seconds)

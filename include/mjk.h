@@ -134,6 +134,10 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n
 int mjk_profile(mjk_ctx* ctx, mjk_basis* basis, int enable);
 int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* basis, int max_classes, int* cls, double* ms,
                      long long* nq, long long* nprimq, double* flops, long long* launches);
+/* Kernel each profiled class last ran on, in mjk_profile_read's order:
+ * 0 tile kernel (k_jk), 1 row kernel (k_jk_row), 2 cooperative kernel
+ * (k_jk_coop).  Returns the number of classes written. */
+int mjk_profile_kinds(mjk_ctx* ctx, const mjk_basis* basis, int max_classes, int* kind);
 
 /* Device memory helpers (for callers without another GPU runtime). */
 int mjk_device_alloc(mjk_ctx* ctx, size_t bytes, void** ptr);

@@ -33,18 +33,18 @@
 
 namespace mjk {
 
-// Cartesian exponents in the order lx = l..0, ly = l-lx..0.
-__host__ __device__ constexpr int cart_x(int l, int i)
+// Cartesian component i of angular momentum l, in the order xx..x, then
+// decreasing lx, and within one lx decreasing ly: i = r (r + 1) / 2 + lz with
+// r = l - lx = ly + lz.  Branch-free closed forms (i < 21, l <= 5), so that
+// indices built from unrolled loop counters fold to constants.
+__host__ __device__ constexpr int cart_row(int i)
 {
-    int lx = l, k = i;
-    while (k > l - lx) { k -= l - lx + 1; --lx; }
-    return lx;
+    return (i >= 1) + (i >= 3) + (i >= 6) + (i >= 10) + (i >= 15);
 }
+__host__ __device__ constexpr int cart_x(int l, int i) { return l - cart_row(i); }
 __host__ __device__ constexpr int cart_y(int l, int i)
 {
-    int lx = l, k = i;
-    while (k > l - lx) { k -= l - lx + 1; --lx; }
-    return l - lx - k;
+    return cart_row(i) - (i - cart_row(i) * (cart_row(i) + 1) / 2);
 }
 __host__ __device__ constexpr int cart_z(int l, int i) { return l - cart_x(l, i) - cart_y(l, i); }
 
@@ -171,11 +171,15 @@ __device__ __forceinline__ void count_work(unsigned long long* c, bool alive, in
 // index stays a constant and the slice's integrals live in registers.  The
 // roots and the vertical recurrence are recomputed per slice.
 
-// Slice geometry: depth 0 = all components, 1 = fixed a, 2 = fixed (a, b),
+// Slice geometry: depth 0 = all components, 1 = a block of NA / SPLIT
+// consecutive a components (one a when SPLIT = NA), 2 = fixed (a, b),
 // 3 = fixed (a, b, c).
 template <int NA, int NB, int NC, int SPLIT>
 struct SliceShape {
-    static constexpr int depth = SPLIT == 1 ? 0 : SPLIT == NA ? 1 : SPLIT == NA * NB ? 2 : 3;
+    static constexpr int depth = SPLIT == 1                      ? 0
+                                 : (SPLIT <= NA && NA % SPLIT == 0) ? 1
+                                 : SPLIT == NA * NB                 ? 2
+                                                                    : 3;
 };
 
 struct QuartetCtx {
@@ -202,8 +206,8 @@ template <int LA, int LB, int LC, int LD, int SPLIT, int S>
 struct SliceRange {
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
     static constexpr int DEPTH = SliceShape<NA, NB, NC, SPLIT>::depth;
-    static constexpr int IA0 = DEPTH == 0 ? 0 : DEPTH == 1 ? S : DEPTH == 2 ? S / NB : S / (NB * NC);
-    static constexpr int IA1 = DEPTH == 0 ? NA : IA0 + 1;
+    static constexpr int IA0 = DEPTH == 0 ? 0 : DEPTH == 1 ? S * (NA / SPLIT) : DEPTH == 2 ? S / NB : S / (NB * NC);
+    static constexpr int IA1 = DEPTH == 0 ? NA : DEPTH == 1 ? IA0 + NA / SPLIT : IA0 + 1;
     static constexpr int IB0 = DEPTH <= 1 ? 0 : DEPTH == 2 ? S % NB : (S / NC) % NB;
     static constexpr int IB1 = DEPTH <= 1 ? NB : IB0 + 1;
     static constexpr int IC0 = DEPTH <= 2 ? 0 : S % NC;

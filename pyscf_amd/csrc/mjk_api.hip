@@ -36,6 +36,7 @@ namespace mjk {
 int launch_jk(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, hipStream_t st);
 int launch_jk_rows(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, size_t shmem, hipStream_t st);
 int row_kernel_available(int la, int lb, int lc, int ld);
+int coop_kernel(int la, int lb, int lc, int ld);
 int launch_qcond(int la, int lb, const QCondArgs& a, hipStream_t st);
 int upload_c2s(hipStream_t st);
 int launch_sph2cart(const XformArgs& x, int maxld, const double* Dsph, long long sslot, int n_dm,
@@ -235,6 +236,7 @@ struct mjk_basis {
     std::vector<hipEvent_t> pev;          // 2 per quartet class
     std::vector<double> prof_ms;
     std::vector<long long> prof_nq, prof_np, prof_launches;
+    std::vector<int> prof_kind;           // kernel of the class's last launch: 0 tile, 1 row, 2 cooperative
 };
 
 namespace {
@@ -1001,6 +1003,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         }
         if (rr != 0 && launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
             return fail(ctx, MJK_EUNSUPPORTED, "no kernel for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
+        if (prof) b->prof_kind[c] = rr == 0 ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
         if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c + 1], st));
     }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
@@ -1098,6 +1101,7 @@ int mjk_profile(mjk_ctx* ctx, mjk_basis* b, int enable)
     b->prof_nq.assign(n, 0);
     b->prof_np.assign(n, 0);
     b->prof_launches.assign(n, 0);
+    b->prof_kind.assign(n, -1);
     if (enable < 0 || enable > 2) return fail(ctx, MJK_EINVAL, "profile mode must be 0, 1 or 2");
     b->profiling = enable != 0;
     b->prof_counts = enable == 1;
@@ -1123,6 +1127,18 @@ int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* b, int max_classes, int* cls
         flops[k] = flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)b->prof_nq[c], (double)b->prof_np[c]);
         launches[k] = b->prof_launches[c];
         ++k;
+    }
+    return k;
+}
+
+int mjk_profile_kinds(mjk_ctx* ctx, const mjk_basis* b, int max_classes, int* kind)
+{
+    if (!ctx || !b || !kind) return MJK_EINVAL;
+    int k = 0;
+    for (size_t c = 0; c < b->qcls.size() && c < b->prof_ms.size(); ++c) {
+        if (b->prof_launches[c] == 0) continue;
+        if (k >= max_classes) break;
+        kind[k++] = b->prof_kind[c];
     }
     return k;
 }

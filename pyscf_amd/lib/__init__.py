@@ -62,6 +62,7 @@ def _declare(lib):
     lib.mjk_rys_roots.argtypes = [c_int, c_double, c_void_p, c_void_p]
     lib.mjk_profile.argtypes = [c_void_p, c_void_p, c_int]
     lib.mjk_profile_read.argtypes = [c_void_p, c_void_p, c_int] + [c_void_p] * 6
+    lib.mjk_profile_kinds.argtypes = [c_void_p, c_void_p, c_int, c_void_p]
     for name in ('mjk_int1e',):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]
