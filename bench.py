@@ -32,6 +32,14 @@ METRIC = 'JK builds/s + ERI quartets/s, mixed vs FP64, at <=1e-8 Eh; p50/p99 lat
 FP64_PEAK = 78.6e12
 
 
+def progress(msg):
+    """One progress line on stderr (the JSON line alone goes to stdout)."""
+    print('[bench %.0fs] %s' % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
+
+
+_T0 = time.perf_counter()
+
+
 def kernel_name(cls, kind):
     """Demangled name (as rocprofv3 prints it, less the argument list) of the
     kernel a quartet class ran on: 0 tile, 1 row, 2 cooperative; classes cut
@@ -119,18 +127,20 @@ def bench_batched(args):
     naos = [m.nao_nr() for m in mols]
     res = {}
     for label, tol32 in (('fp64', 0.0), ('mixed', args.fp32_tol)):
-        b = batch.BatchRHF(mols, fp32_tol=tol32)
+        progress('c4 %s: batched RHF of %d molecules' % (label, len(mols)))
+        b = batch.BatchRHF(mols, fp32_tol=tol32, log=lambda s, label=label: progress('c4 %s %s' % (label, s)))
         b.conv_tol = 1e-9
         t0 = time.perf_counter()
         e = b.kernel()
         t_scf = time.perf_counter() - t0
         t_jk = sum(b.build_times)
         res[label] = {'e_tot': e, 'scf_s': t_scf, 'iterations': b.cycles, 'jk_s': t_jk,
-                      'converged': int(b.converged.sum()),
+                      'converged': int(b.converged.sum()), 'conv_mask': numpy.array(b.converged),
                       'mol_builds_per_s': len(mols) * len(b.build_times) / t_jk}
     # unbatched baseline: the FP64 J/K build of every molecule, one call each
     dms = b.dms
     eng = _vhf.get_engine()
+    progress('c4: unbatched baseline, %d uploads' % len(mols))
     opts = [_vhf.VHFOpt(m, direct_scf_tol=1e-13, engine=eng) for m in mols]
     for o, d in zip(opts, dms):
         o.basis.get_jk(d[None], 1, True, True, 1e-13)
@@ -140,6 +150,7 @@ def bench_batched(args):
     t_single = time.perf_counter() - t0
     single = len(mols) / t_single
     de = numpy.abs(res['mixed']['e_tot'] - res['fp64']['e_tot'])
+    both = res['mixed']['conv_mask'] & res['fp64']['conv_mask']
     out = {
         'metric': METRIC, 'value': res['mixed']['mol_builds_per_s'], 'unit': 'molecule JK builds/s',
         'n_gpus': 1, 'steps': res['mixed']['iterations'], 'warmup': 0,
@@ -152,7 +163,8 @@ def bench_batched(args):
         'unbatched_fp64_mol_builds_per_s': single,
         'speedup_mixed_batched_vs_unbatched_fp64': res['mixed']['mol_builds_per_s'] / single,
         'speedup_fp64_batched_vs_unbatched_fp64': res['fp64']['mol_builds_per_s'] / single,
-        'max_abs_dE_mixed_vs_fp64': float(de.max()),
+        'max_abs_dE_mixed_vs_fp64': float(de[both].max()) if both.any() else None,
+        'max_abs_dE_mixed_vs_fp64_incl_unconverged': float(de.max()),
         'converged': [res['fp64']['converged'], res['mixed']['converged']],
         'scf_seconds': [res['fp64']['scf_s'], res['mixed']['scf_s']],
     }
@@ -179,6 +191,7 @@ def bench_serving(args):
             dm = c @ c.T
         reqs.append((kind, mol, dm, basis))
     gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
+    progress('c5: %d requests built' % len(reqs))
     srv = JKServer(nslots=8)
     t0 = time.perf_counter()
     futs = []
@@ -186,12 +199,13 @@ def bench_serving(args):
         time.sleep(g)
         futs.append(srv.submit(Request(kind, mol, dm)))
     ok = 0
-    for f in futs:
+    for i, f in enumerate(futs):
         try:
             f.result(timeout=1200)
             ok += 1
-        except Exception:
-            pass
+        except Exception as e:
+            progress('c5: request %d failed: %s' % (i, e))
+        progress('c5: %d of %d requests done' % (i + 1, len(futs)))
     t_all = time.perf_counter() - t0
     srv.close()
     lat = numpy.array([r.t_done - r.t_arrival for r in srv.finished])

@@ -38,6 +38,14 @@ struct PairSet {
     const int* shB;
     const int* cartA;      // Cartesian AO offset of A within its molecule
     const int* cartB;
+    // Ket-side copy of the primitive pairs, transposed per (class, molecule,
+    // bucket) segment: primitive ik of pair q sits at tP[tpos[q] + ik * tstride[q]]
+    // (tstride = the segment's pair count), so lanes holding consecutive kets
+    // read consecutive addresses.
+    const double4* tP;
+    const double2* tK;
+    const int* tpos;
+    const int* tstride;
 };
 
 struct MolSet {

@@ -121,6 +121,19 @@ __device__ __forceinline__ void atomic_add(double* p, double v)
     unsafeAtomicAdd(p, v);
 }
 
+// Load of read-only data at a wave-uniform address through the scalar cache
+// (constant address space: s_load into SGPRs), which keeps block-uniform pair
+// data out of the vector registers.
+template <typename T>
+__device__ __forceinline__ T uload(const T* p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T*)(p);
+#else
+    return *p;
+#endif
+}
+
 // Locate the bra tile of this block: largest i with bt_off[i] <= b.
 __device__ __forceinline__ int find_tile(const int* __restrict__ off, int n, int b)
 {
@@ -187,6 +200,7 @@ struct QuartetCtx {
     bool alive, pvalid, qvalid;
     double4 rA, rC, AB, CD;
     int pb0, npb, pk0, npk;
+    int tq, ts;          // ket primitives in the transposed copy: tP[tq + ik * ts]
     double fac;
 };
 
@@ -216,71 +230,127 @@ struct SliceRange {
     static constexpr int NFL = MA * MB * MC * ND;
 };
 
+// One primitive quartet: what the root loop needs.
+struct PrimQuartet {
+    double ipq, pref, p, q, ip2, iq2;     // 1/(p+q), prefactor, p, q, 1/(2p), 1/(2q)
+    double PAx, PAy, PAz, QCx, QCy, QCz, PQx, PQy, PQz;
+};
+
+__device__ __forceinline__ PrimQuartet prim_quartet(const double4& Pp, const double2& Kp, const double4& Qp,
+                                                    const double2& Kq, const QuartetCtx& c, double& T)
+{
+    PrimQuartet m;
+    const double pq = Pp.w + Qp.w;
+    const double sipq = rsqrt_nr(pq);   // 1/sqrt(p+q)
+    m.ipq = sipq * sipq;
+    m.PQx = Pp.x - Qp.x;
+    m.PQy = Pp.y - Qp.y;
+    m.PQz = Pp.z - Qp.z;
+    T = Pp.w * Qp.w * m.ipq * (m.PQx * m.PQx + m.PQy * m.PQy + m.PQz * m.PQz);
+    m.pref = Kp.x * Kq.x * sipq;
+    m.p = Pp.w;
+    m.q = Qp.w;
+    m.ip2 = 0.5 * Kp.y;
+    m.iq2 = 0.5 * Kq.y;
+    m.PAx = Pp.x - c.rA.x;
+    m.PAy = Pp.y - c.rA.y;
+    m.PAz = Pp.z - c.rA.z;
+    m.QCx = Qp.x - c.rC.x;
+    m.QCy = Qp.y - c.rC.y;
+    m.QCz = Qp.z - c.rC.z;
+    return m;
+}
+
+// Adds root (rx, rw) of primitive quartet m to the slice's Cartesian block.
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R>
+__device__ __forceinline__ void add_root(const PrimQuartet& m, double rx, double rw, const QuartetCtx& c,
+                                         R (&gout)[SliceRange<LA, LB, LC, LD, SPLIT, S>::NFL])
+{
+    using SR = SliceRange<LA, LB, LC, LD, SPLIT, S>;
+    constexpr int ND = SR::ND;
+    constexpr int NI = LA + LB + 1, NK = LC + LD + 1;
+    constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
+    constexpr int MB = SR::MB, MC = SR::MC;
+    const double u = rx * m.ipq;
+    const double uq = u * m.q, up = u * m.p;
+    const R b10 = R(m.ip2 * (1.0 - uq)), b01 = R(m.iq2 * (1.0 - up)), b00 = R(0.5 * u);
+    R gx[NI][NK], gy[NI][NK], gz[NI][NK];
+    vrr<NI, NK, R>(gx, R(1), R(m.PAx - uq * m.PQx), R(m.QCx + up * m.PQx), b10, b01, b00);
+    vrr<NI, NK, R>(gy, R(1), R(m.PAy - uq * m.PQy), R(m.QCy + up * m.PQy), b10, b01, b00);
+    vrr<NI, NK, R>(gz, R(rw * m.pref), R(m.PAz - uq * m.PQz), R(m.QCz + up * m.PQz), b10, b01, b00);
+    R Ix[LA + 1][LB + 1][LC + 1][LD + 1];
+    R Iy[LA + 1][LB + 1][LC + 1][LD + 1];
+    R Iz[LA + 1][LB + 1][LC + 1][LD + 1];
+    hrr<LA, LB, LC, LD, R>(gx, R(c.AB.x), R(c.CD.x), Ix);
+    hrr<LA, LB, LC, LD, R>(gy, R(c.AB.y), R(c.CD.y), Iy);
+    hrr<LA, LB, LC, LD, R>(gz, R(c.AB.z), R(c.CD.z), Iz);
+#pragma unroll
+    for (int ia = IA0; ia < IA1; ++ia)
+#pragma unroll
+    for (int ib2 = IB0; ib2 < IB1; ++ib2)
+#pragma unroll
+    for (int ic = IC0; ic < IC1; ++ic)
+#pragma unroll
+    for (int id = 0; id < ND; ++id) {
+        const int f = (((ia - IA0) * MB + ib2 - IB0) * MC + ic - IC0) * ND + id;
+        gout[f] = fma(Ix[cart_x(LA, ia)][cart_x(LB, ib2)][cart_x(LC, ic)][cart_x(LD, id)]
+                          * Iy[cart_y(LA, ia)][cart_y(LB, ib2)][cart_y(LC, ic)][cart_y(LD, id)],
+                      Iz[cart_z(LA, ia)][cart_z(LB, ib2)][cart_z(LC, ic)][cart_z(LD, id)], gout[f]);
+    }
+}
+
+// Ket primitives evaluated two at a time (independent chains for the
+// scheduler to interleave) in classes whose slice is at most this many
+// components; larger slices keep one at a time for their registers.
+#ifndef MJK_PAIR_NFL
+#define MJK_PAIR_NFL 27
+#endif
+
 // Contracted Cartesian block of one quartet restricted to slice S, times
 // the quartet's degeneracy factor, in FP64 (recurrences in precision R).
-template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R>
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R, bool UNIFORM_BRA = false>
 __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
                                           double (&gd)[SliceRange<LA, LB, LC, LD, SPLIT, S>::NFL])
 {
     using SR = SliceRange<LA, LB, LC, LD, SPLIT, S>;
-    constexpr int ND = SR::ND;
     constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
-    constexpr int NI = LA + LB + 1, NK = LC + LD + 1;
-    constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
-    constexpr int MB = SR::MB, MC = SR::MC, NFL = SR::NFL;
+    constexpr int NFL = SR::NFL;
 
     R gout[NFL];
 #pragma unroll
     for (int f = 0; f < NFL; ++f) gout[f] = R(0);
 
     for (int ib = 0; ib < c.npb; ++ib) {
-        const double4 Pp = a.P.ppP[c.pb0 + ib];
-        const double2 Kp = a.P.ppK[c.pb0 + ib];
-        const double PAx = Pp.x - c.rA.x, PAy = Pp.y - c.rA.y, PAz = Pp.z - c.rA.z;
-        for (int ik = 0; ik < c.npk; ++ik) {
-            const double4 Qp = a.P.ppP[c.pk0 + ik];
-            const double2 Kq = a.P.ppK[c.pk0 + ik];
-            const double pq = Pp.w + Qp.w;
-            const double ipq = 1.0 / pq;
-            const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
-            const double T = Pp.w * Qp.w * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
-            const double pref = Kp.x * Kq.x * sqrt(ipq);
+        const double4 Pp = UNIFORM_BRA ? uload(a.P.ppP + c.pb0 + ib) : a.P.ppP[c.pb0 + ib];
+        const double2 Kp = UNIFORM_BRA ? uload(a.P.ppK + c.pb0 + ib) : a.P.ppK[c.pb0 + ib];
+        int ik = 0;
+        if (NFL <= MJK_PAIR_NFL) {
+            for (; ik + 1 < c.npk; ik += 2) {
+                double T0, T1;
+                const int t0 = c.tq + ik * c.ts, t1 = t0 + c.ts;
+                const PrimQuartet m0 = prim_quartet(Pp, Kp, a.P.tP[t0], a.P.tK[t0], c, T0);
+                const PrimQuartet m1 = prim_quartet(Pp, Kp, a.P.tP[t1], a.P.tK[t1], c, T1);
+                const RysCursor<NR> rys0(a.rys, T0), rys1(a.rys, T1);
+#pragma unroll 1
+                for (int r = 0; r < NR; ++r) {
+                    double rx0, rw0, rx1, rw1;
+                    rys0.root(r, rx0, rw0);
+                    rys1.root(r, rx1, rw1);
+                    add_root<LA, LB, LC, LD, SPLIT, S, R>(m0, rx0, rw0, c, gout);
+                    add_root<LA, LB, LC, LD, SPLIT, S, R>(m1, rx1, rw1, c, gout);
+                }
+            }
+        }
+        for (; ik < c.npk; ++ik) {
+            double T;
+            const int t0 = c.tq + ik * c.ts;
+            const PrimQuartet m = prim_quartet(Pp, Kp, a.P.tP[t0], a.P.tK[t0], c, T);
             const RysCursor<NR> rys(a.rys, T);
-            const double QCx = Qp.x - c.rC.x, QCy = Qp.y - c.rC.y, QCz = Qp.z - c.rC.z;
 #pragma unroll 1
             for (int r = 0; r < NR; ++r) {
                 double rx, rw;
                 rys.root(r, rx, rw);
-                const double u = rx * ipq;
-                const double B00 = 0.5 * u;
-                const double B10 = 0.5 * Kp.y * (1.0 - u * Qp.w);
-                const double B01 = 0.5 * Kq.y * (1.0 - u * Pp.w);
-                const double uq = u * Qp.w, up = u * Pp.w;
-                R gx[NI][NK], gy[NI][NK], gz[NI][NK];
-                const R b10 = R(B10), b01 = R(B01), b00 = R(B00);
-                vrr<NI, NK, R>(gx, R(1), R(PAx - uq * PQx), R(QCx + up * PQx), b10, b01, b00);
-                vrr<NI, NK, R>(gy, R(1), R(PAy - uq * PQy), R(QCy + up * PQy), b10, b01, b00);
-                vrr<NI, NK, R>(gz, R(rw * pref), R(PAz - uq * PQz), R(QCz + up * PQz), b10, b01, b00);
-                R Ix[LA + 1][LB + 1][LC + 1][LD + 1];
-                R Iy[LA + 1][LB + 1][LC + 1][LD + 1];
-                R Iz[LA + 1][LB + 1][LC + 1][LD + 1];
-                hrr<LA, LB, LC, LD, R>(gx, R(c.AB.x), R(c.CD.x), Ix);
-                hrr<LA, LB, LC, LD, R>(gy, R(c.AB.y), R(c.CD.y), Iy);
-                hrr<LA, LB, LC, LD, R>(gz, R(c.AB.z), R(c.CD.z), Iz);
-#pragma unroll
-                for (int ia = IA0; ia < IA1; ++ia)
-#pragma unroll
-                for (int ib2 = IB0; ib2 < IB1; ++ib2)
-#pragma unroll
-                for (int ic = IC0; ic < IC1; ++ic)
-#pragma unroll
-                for (int id = 0; id < ND; ++id) {
-                    const int f = (((ia - IA0) * MB + ib2 - IB0) * MC + ic - IC0) * ND + id;
-                    gout[f] = fma(Ix[cart_x(LA, ia)][cart_x(LB, ib2)][cart_x(LC, ic)][cart_x(LD, id)]
-                                      * Iy[cart_y(LA, ia)][cart_y(LB, ib2)][cart_y(LC, ic)][cart_y(LD, id)],
-                                  Iz[cart_z(LA, ia)][cart_z(LB, ib2)][cart_z(LC, ic)][cart_z(LD, id)],
-                                  gout[f]);
-                }
+                add_root<LA, LB, LC, LD, SPLIT, S, R>(m, rx, rw, c, gout);
             }
         }
     }
@@ -454,6 +524,7 @@ __global__ __launch_bounds__(64) void k_jk(const JKArgs a)
     c.alive = c.pvalid && c.qvalid && !(a.diag && c.q > c.p);
     c.rA = c.rC = c.AB = c.CD = make_double4(0, 0, 0, 0);
     c.pb0 = c.npb = c.pk0 = c.npk = 0;
+    c.tq = c.ts = 0;
     bool small = true;
     if (c.alive) {
         c.rA = a.P.rA[c.p];
@@ -472,6 +543,8 @@ __global__ __launch_bounds__(64) void k_jk(const JKArgs a)
         c.npb = a.P.nprim[c.p];
         c.pk0 = a.P.prim0[c.q];
         c.npk = a.P.nprim[c.q];
+        c.tq = a.P.tpos[c.q];
+        c.ts = a.P.tstride[c.q];
         c.AB = a.P.rAB[c.p];
         c.CD = a.P.rAB[c.q];
         double f = 1.0;
@@ -640,7 +713,8 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
                     const int r = it / 3;
                     const int d = it - 3 * r;
                     const double pq = Pp.w + Qp.w;
-                    const double ipq = 1.0 / pq;
+                    const double sipq = rsqrt_nr(pq);
+                    const double ipq = sipq * sipq;
                     const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
                     const double T = Pp.w * Qp.w * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
                     const RysCursor<NR> rys(a.rys, T);
@@ -652,7 +726,7 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
                     const double QCd = d == 0 ? Qp.x - rC.x : d == 1 ? Qp.y - rC.y : Qp.z - rC.z;
                     const double ABd = d == 0 ? AB.x : d == 1 ? AB.y : AB.z;
                     const double CDd = d == 0 ? CD.x : d == 1 ? CD.y : CD.z;
-                    const double g00 = d == 2 ? rw * Kp.x * Kq.x * sqrt(ipq) : 1.0;
+                    const double g00 = d == 2 ? rw * Kp.x * Kq.x * sipq : 1.0;
                     double g[NI][NK];
                     vrr<NI, NK, double>(g, g00, PAd - u * Qp.w * PQd, QCd + u * Pp.w * PQd,
                                         0.5 * Kp.y * (1.0 - u * Qp.w), 0.5 * Kq.y * (1.0 - u * Pp.w), 0.5 * u);
@@ -812,7 +886,7 @@ __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c,
     constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
     constexpr int MA = SR::MA, MB = SR::MB, MC = SR::MC;
     double gd[SR::NFL];
-    eri_slice<LA, LB, LC, LD, SPLIT, S, R>(a, c, gd);
+    eri_slice<LA, LB, LC, LD, SPLIT, S, R, true>(a, c, gd);
     const int oc = c.qvalid ? a.P.cartA[c.q] : 0, od = c.qvalid ? a.P.cartB[c.q] : 0;
     const int lane = threadIdx.x & 63;
     for (int s = 0; s < a.n_j; ++s) {
@@ -952,17 +1026,17 @@ void k_jk_row(const JKArgs a)
     __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
     __shared__ int s_next;   // next unclaimed ket of the chunk (waves take 64 at a time)
     const int tid = threadIdx.x, lane = tid & 63;
-    const int4 task = a.rt[blockIdx.x];
+    const int4 task = uload(a.rt + blockIdx.x);
     const int p = task.x;
-    const int mol = a.pmol[p];
-    const int ld = a.M.ld[mol];
-    const long long base = a.M.dbase[mol];
+    const int mol = uload(a.pmol + p);
+    const int ld = uload(a.M.ld + mol);
+    const long long base = uload(a.M.dbase + mol);
     const int krow = NROW * ld;
     double* kr = lds_rows;
     double* jr = lds_rows + a.n_k * krow;
     const int nz = a.n_k * krow + a.n_j * NA * NB;
     for (int e = tid; e < nz; e += 256) lds_rows[e] = 0.0;
-    const double qp = a.P.rA[p].w;
+    const double qp = uload(a.P.rA + p).w;
     // surviving ket range of every primitive bucket (Schwarz prefix of the
     // descending segment, q <= p on the diagonal class), from the task list
     if (tid < a.nbucket) {
@@ -983,12 +1057,12 @@ void k_jk_row(const JKArgs a)
     c.p = p;
     c.mol = mol;
     c.pvalid = true;
-    c.rA = a.P.rA[p];
-    c.AB = a.P.rAB[p];
-    c.pb0 = a.P.prim0[p];
-    c.npb = a.P.nprim[p];
-    const bool diagA = a.P.shA[p] == a.P.shB[p];
-    const int oa = a.P.cartA[p], ob = a.P.cartB[p];
+    c.rA = uload(a.P.rA + p);
+    c.AB = uload(a.P.rAB + p);
+    c.pb0 = uload(a.P.prim0 + p);
+    c.npb = uload(a.P.nprim + p);
+    const bool diagA = uload(a.P.shA + p) == uload(a.P.shB + p);
+    const int oa = uload(a.P.cartA + p), ob = uload(a.P.cartB + p);
     const int cp = xcc_copy(a.ncopy);
     unsigned long long cnt[4] = {0, 0, 0, 0};
     for (;;) {
@@ -1010,6 +1084,7 @@ void k_jk_row(const JKArgs a)
         c.alive = c.qvalid;
         c.rC = c.CD = make_double4(0, 0, 0, 0);
         c.pk0 = c.npk = 0;
+        c.tq = c.ts = 0;
         bool small = true;
         if (c.alive) {
             c.rC = a.P.rA[c.q];
@@ -1026,6 +1101,8 @@ void k_jk_row(const JKArgs a)
         if (c.alive) {
             c.pk0 = a.P.prim0[c.q];
             c.npk = a.P.nprim[c.q];
+            c.tq = a.P.tpos[c.q];
+            c.ts = a.P.tstride[c.q];
             c.CD = a.P.rAB[c.q];
             double f = diagA ? 0.5 : 1.0;
             if (a.P.shA[c.q] == a.P.shB[c.q]) f *= 0.5;

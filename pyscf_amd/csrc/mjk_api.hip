@@ -208,6 +208,9 @@ struct mjk_basis {
     double4* ppP = nullptr;
     double2* ppK = nullptr;
     int *prim0 = nullptr, *nprim = nullptr, *shA = nullptr, *shB = nullptr, *cartA = nullptr, *cartB = nullptr;
+    double4* tP = nullptr;           // ket-side transposed primitive pairs (PairSet::tP)
+    double2* tK = nullptr;
+    int *tpos = nullptr, *tstride = nullptr;
     double4 *rA = nullptr, *rAB = nullptr;
     std::vector<int> seg;            // [(class * nmol + mol) * NBUCKET + bucket] -> first pair
     int* d_pmol = nullptr;           // molecule of every pair
@@ -421,6 +424,10 @@ PairSet pairset(const mjk_basis* b)
     P.shB = b->shB;
     P.cartA = b->cartA;
     P.cartB = b->cartB;
+    P.tP = b->tP;
+    P.tK = b->tK;
+    P.tpos = b->tpos;
+    P.tstride = b->tstride;
     return P;
 }
 
@@ -627,6 +634,37 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
         if ((rc = track_upload(b, hsb, &b->shB))) return rc;
         if ((rc = track_upload(b, hca, &b->cartA))) return rc;
         if ((rc = track_upload(b, hcb, &b->cartB))) return rc;
+        if (qv) {
+            // ket-side transposed copy: per segment, primitive ik of its pair j at
+            // base + ik * n + j (n pairs, padded to the segment's largest count)
+            std::vector<double4> tP;
+            std::vector<double2> tK;
+            std::vector<int> tpos, tstr;
+            tpos.reserve(hp0.size());
+            tstr.reserve(hp0.size());
+            for (auto& segv : byseg) {
+                const int n = (int)segv.size();
+                size_t mx = 0;
+                for (auto& ph : segv) mx = std::max(mx, ph.P.size());
+                const size_t base = tP.size();
+                tP.resize(base + mx * n, make_double4(0, 0, 0, 1));
+                tK.resize(base + mx * n, make_double2(0, 1));
+                for (int j = 0; j < n; ++j) {
+                    const PairHost& ph = segv[j];
+                    for (size_t ik = 0; ik < ph.P.size(); ++ik) {
+                        tP[base + ik * n + j] = ph.P[ik];
+                        tK[base + ik * n + j] = ph.K[ik];
+                    }
+                    tpos.push_back((int)(base + j));
+                    tstr.push_back(n);
+                }
+            }
+            if (tP.size() > (size_t)2000000000) return fail(ctx, MJK_EUNSUPPORTED, "too many primitive pairs");
+            if ((rc = track_upload(b, tP, &b->tP))) return rc;
+            if ((rc = track_upload(b, tK, &b->tK))) return rc;
+            if ((rc = track_upload(b, tpos, &b->tpos))) return rc;
+            if ((rc = track_upload(b, tstr, &b->tstride))) return rc;
+        }
         b->h_shA = hsa;
         b->h_shB = hsb;
         b->h_nprim = hnp;

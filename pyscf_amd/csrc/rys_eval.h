@@ -15,6 +15,24 @@
 
 namespace mjk {
 
+// 1/sqrt(x) for normal positive x: on the device the hardware estimate
+// (v_rsq_f64) refined by two Newton-Raphson steps y <- y (3 - x y^2) / 2, each
+// of which squares the relative error (written as y + y (1/2 - (x y / 2) y));
+// on the host the library square root.
+__host__ __device__ __forceinline__ double rsqrt_nr(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * x * y;
+    y = fma(y, fma(-h, y, 0.5), y);
+    h = 0.5 * x * y;
+    y = fma(y, fma(-h, y, 0.5), y);
+    return y;
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
 // Offset of the Hermite constants behind the Chebyshev coefficients in the
 // single device buffer holding both.
 constexpr int RYS_HERM_BASE = RYS_TABLE_SIZE;
@@ -72,8 +90,8 @@ struct RysCursor {
     {
         asym = !(T < 36.0 + 6.0 * N);   // also catches NaN: constant indices only
         if (asym) {
-            s = 1.0 / T;
-            st = sqrt(s);
+            st = rsqrt_nr(T);
+            s = st * st;
             c = tab + RYS_HERM_BASE + RYS_HOFF[N];
         } else {
             int iv = (int)(T * 0.5);

@@ -17,8 +17,9 @@ from pyscf_amd.scf import _vhf, rhf
 
 
 class BatchRHF:
-    def __init__(self, mols, direct_scf_tol=1e-13, fp32_tol=0.0, engine=None):
+    def __init__(self, mols, direct_scf_tol=1e-13, fp32_tol=0.0, engine=None, log=None):
         self.mols = list(mols)
+        self.log = log              # optional callable(str): one line per SCF cycle
         self.direct_scf_tol = direct_scf_tol
         self.fp32_tol = fp32_tol
         self.conv_tol = 1e-9
@@ -89,6 +90,9 @@ class BatchRHF:
                 if abs(e_tot[i] - e_last) < self.conv_tol and numpy.linalg.norm(g) < conv_g:
                     active[i] = False
             dms = new_dms
+            if self.log:
+                self.log('cycle %d: %d of %d molecules active, J/K build %.1f ms'
+                         % (cycle + 1, int(active.sum()), len(mols), 1e3 * self.build_times[-1]))
             if not active.any():
                 break
         self.e_tot = numpy.array(e_tot)
