@@ -583,11 +583,15 @@ struct CoopShape {
     static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
     static constexpr int ISZ = (LA + 1) * (LB + 1) * (LC + 1) * (LD + 1);
     static constexpr int SI = NR * 3 * ISZ;
-    static constexpr int SU = SI > NF ? SI : NF;
     static constexpr int NDB = NA * NB + NC * ND + NB * NC + NA * NC + NB * ND + NA * ND;
     static constexpr int NW = NF <= 64 * 24 ? 1 : NF <= 128 * 24 ? 2 : 4;
     static constexpr int NT = 64 * NW;
     static constexpr int CH = (NF + NT - 1) / NT;
+    // primitive quartets per phase: enough (root, direction) items to occupy the
+    // block, within 6144 doubles (48 KB) of I arrays
+    static constexpr int PB0 = NT / (NR * 3) < 6144 / SI ? NT / (NR * 3) : 6144 / SI;
+    static constexpr int PB = PB0 < 1 ? 1 : PB0;
+    static constexpr int SU = PB * SI > NF ? PB * SI : NF;
 };
 
 // Transfer recurrence g(a+b, c+d) -> I(a, b, c, d) written to LDS as
@@ -703,15 +707,20 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
 #pragma unroll
         for (int c = 0; c < CH; ++c) gout[c] = 0.0;
 
-        for (int ib = 0; ib < npb; ++ib) {
-            const double4 Pp = a.P.ppP[pb0 + ib];
-            const double2 Kp = a.P.ppK[pb0 + ib];
-            for (int ik = 0; ik < npk; ++ik) {
-                const double4 Qp = a.P.ppP[pk0 + ik];
-                const double2 Kq = a.P.ppK[pk0 + ik];
-                for (int it = tid; it < NITEM; it += NT) {
-                    const int r = it / 3;
-                    const int d = it - 3 * r;
+        constexpr int PB = Sh::PB;
+        const int npq = npb * npk;
+        for (int j0 = 0; j0 < npq; j0 += PB) {      // PB primitive quartets per phase
+            const int nb = min(PB, npq - j0);
+            {
+                for (int it = tid; it < nb * NITEM; it += NT) {
+                    const int j = it / NITEM;
+                    const int r = (it - j * NITEM) / 3;
+                    const int d = it - j * NITEM - 3 * r;
+                    const int ib = (j0 + j) / npk, ik = j0 + j - ib * npk;
+                    const double4 Pp = a.P.ppP[pb0 + ib];
+                    const double2 Kp = a.P.ppK[pb0 + ib];
+                    const double4 Qp = a.P.ppP[pk0 + ik];
+                    const double2 Kq = a.P.ppK[pk0 + ik];
                     const double pq = Pp.w + Qp.w;
                     const double sipq = rsqrt_nr(pq);
                     const double ipq = sipq * sipq;
@@ -736,10 +745,12 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
 #pragma unroll
                 for (int c = 0; c < CH; ++c) {
                     double s2 = 0.0;
+                    for (int j = 0; j < nb; ++j) {
 #pragma unroll
-                    for (int r = 0; r < NR; ++r) {
-                        const double* Ir = sblk + r * 3 * ISZ;
-                        s2 = fma(Ir[ox[c]] * Ir[oy[c]], Ir[oz[c]], s2);
+                        for (int r = 0; r < NR; ++r) {
+                            const double* Ir = sblk + (j * NR + r) * 3 * ISZ;
+                            s2 = fma(Ir[ox[c]] * Ir[oy[c]], Ir[oz[c]], s2);
+                        }
                     }
                     gout[c] += s2;
                 }
