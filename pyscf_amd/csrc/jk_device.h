@@ -859,7 +859,8 @@ __host__ __device__ constexpr int jk_split(int la, int lb, int lc, int ld)
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
     const int nf = na * nb * nc * nd;
     if (nf <= 81) return 1;
-    if (nf / na <= 54) return na;
+    for (int s = 2; s <= na; ++s)   // fewest blocks of a components with at most 54 components each
+        if (na % s == 0 && nf / s <= 54) return s;
     if (nf / (na * nb) <= 54) return na * nb;
     return na * nb * nc;
 }
