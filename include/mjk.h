@@ -75,9 +75,10 @@ size_t mjk_ctx_hbm_used(const mjk_ctx* ctx);
 /* Kernel selection knob (no counterpart in the reference, whose loop has a
  * single CPU code path): the largest LDS allocation, in bytes, that a
  * row-kernel block may take (one block per bra pair accumulating its K'
- * rows in LDS).  Classes whose rows do not fit run the tile kernel; 0 runs
- * tile kernels only.  Default 61440; at most 61440.  Results agree either
- * way to rounding. */
+ * rows in LDS).  Classes whose rows do not fit run their fallback kernel
+ * (the tile kernel up to 81 Cartesian components per quartet, the
+ * cooperative kernel above); 0 runs the fallback kernels only.  Default
+ * 61440; at most 61440.  Results agree either way to rounding. */
 int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes);
 
 /* Upload one molecule's basis and build its shell pairs, Schwarz bounds
