@@ -157,8 +157,8 @@ def test_cooperative_classes_dd(oracle):
 
 @pytest.mark.parametrize('row_lds', [0, 3072])
 def test_row_and_tile_kernels_agree(oracle, row_lds):
-    # row kernels (K' rows in LDS) against tile kernels: 0 forces tile kernels
-    # everywhere, 3072 bytes keeps row kernels only for the narrowest classes
+    # row kernels (K' rows in LDS) against their fallback kernels: 0 forces the fallback (tile or
+    # cooperative) everywhere, 3072 bytes keeps row kernels only for the narrowest classes
     mol = gto.M(atom='''O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587
                         O 3.1 0.2 0; H 3.1 -0.557 0.587; H 3.1 0.957 0.587''', basis='def2-svp')
     nao = mol.nao_nr()
