@@ -84,9 +84,13 @@ struct JKArgs {
     unsigned long long* counters;  // {FP64 quartets, FP64 primitive quartets, FP32 quartets, FP32 primitive quartets}
     // row kernel: block b takes bra pair rt[b].x and the kets rt[b].y .. min(rt[b].z, n) - 1 of
     // the n surviving kets of the ket class, i.e. of the concatenated ket ranges
-    // rr[rt[b].w * nbucket + k] = {first, end} (one per primitive bucket k)
+    // rr[rt[b].w * nbucket + k] = {first, end} (one per primitive bucket k), for the bra's
+    // primitive pairs rtp[b] .. rtp[b] + rprim - 1 (the integrals are linear in the primitive
+    // sum, so a bra pair's primitives may be shared among several tasks)
     const int4* rt;
     const int2* rr;
+    const int* rtp;
+    int rprim;
     const int* pmol;        // molecule of every pair
     int nbucket;
 };

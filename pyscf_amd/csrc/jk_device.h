@@ -1071,8 +1071,11 @@ void k_jk_row(const JKArgs a)
     c.pvalid = true;
     c.rA = uload(a.P.rA + p);
     c.AB = uload(a.P.rAB + p);
-    c.pb0 = uload(a.P.prim0 + p);
-    c.npb = uload(a.P.nprim + p);
+    {
+        const int pbo = uload(a.rtp + blockIdx.x);   // this task's share of the bra's primitive pairs
+        c.pb0 = uload(a.P.prim0 + p) + pbo;
+        c.npb = min(a.rprim, uload(a.P.nprim + p) - pbo);
+    }
     const bool diagA = uload(a.P.shA + p) == uload(a.P.shB + p);
     const int oa = uload(a.P.cartA + p), ob = uload(a.P.cartB + p);
     const int cp = xcc_copy(a.ncopy);

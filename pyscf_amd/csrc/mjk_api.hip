@@ -98,6 +98,8 @@ struct QClass {
     int nrt = 0;
     int4* rt = nullptr;
     int2* rr = nullptr;
+    int* rtp = nullptr;      // bra primitive offset of every task
+    int rprim = 0;           // bra primitive pairs per task
 };
 
 
@@ -132,6 +134,7 @@ struct mjk_ctx {
     int row_chunk = 16384;    // least primitive quartets per row-kernel block
     int row_blocks = 16384;   // row-kernel blocks per class launch to aim for ...
     int row_min_blocks = 512;    // ... and the fewest, whatever row_chunk asks
+    int row_prim = 0;            // bra primitive pairs per row task (0: by class size)
     hipStream_t stream = nullptr;
     size_t budget = 0, used = 0;
     std::string err;
@@ -252,6 +255,7 @@ void free_tiles(mjk_basis* b)
         dfree(b->ctx, c.bt_off);
         dfree(b->ctx, c.rt);
         dfree(b->ctx, c.rr);
+        dfree(b->ctx, c.rtp);
     }
     b->qcls.clear();
     b->tile_cutoff = -1.0;
@@ -360,9 +364,16 @@ int build_tiles(mjk_basis* b, double cutoff)
                 const long long budget = std::min(std::max(cost_all / ctx->row_blocks, (long long)ctx->row_chunk),
                                                   std::max(cost_all / ctx->row_min_blocks, 1LL));
                 std::vector<std::pair<long long, int4>> tasks;
-                for (size_t ib = 0; ib < bra.size(); ++ib) {
+                std::vector<int> tprim;   // bra primitive offset of every task (parallel to tasks)
+                // bra primitive pairs per task: small classes share their bra pairs out
+                // finely (a small molecule's long contractions are its critical path)
+                const int P = ctx->row_prim > 0 ? ctx->row_prim : cost_all < (1LL << 20) ? 2 : 4;
+                qc.rprim = P;
+                for (size_t ib = 0; ib < bra.size(); ++ib)
+                for (int pbo = 0; pbo < b->h_nprim[bra[ib]]; pbo += P) {
+                    // bra pair ib, its primitive pairs [pbo, pbo + P)
                     const int p = bra[ib];
-                    const long long npb = b->h_nprim[p];
+                    const long long npb = std::min(P, b->h_nprim[p] - pbo);
                     int flat = 0, start = 0;
                     long long cur = 0;
                     for (int k = 0; k < NBUCKET; ++k) {
@@ -385,24 +396,34 @@ int build_tiles(mjk_basis* b, double cutoff)
                             flat += lo - y;
                             y = lo;
                             tasks.push_back({cur, make_int4(p, start, flat, (int)ib)});
+                            tprim.push_back(pbo);
                             start = flat;
                             cur = 0;
                         }
                     }
-                    if (flat > start) tasks.push_back({cur, make_int4(p, start, flat, (int)ib)});
-                    tasks.back().second.z = INT_MAX;   // the bra pair's last chunk runs to its end
+                    if (flat > start) {
+                        tasks.push_back({cur, make_int4(p, start, flat, (int)ib)});
+                        tprim.push_back(pbo);
+                    }
+                    tasks.back().second.z = INT_MAX;   // the last chunk runs to the end of the kets
                 }
                 // heaviest tasks first (the last chunk of a bra pair is the short one)
-                std::stable_sort(tasks.begin(), tasks.end(),
-                                 [](const std::pair<long long, int4>& u, const std::pair<long long, int4>& v) {
-                                     return u.first > v.first;
-                                 });
+                std::vector<size_t> order(tasks.size());
+                std::iota(order.begin(), order.end(), 0);
+                std::stable_sort(order.begin(), order.end(),
+                                 [&](size_t u, size_t v) { return tasks[u].first > tasks[v].first; });
                 if (tasks.size() > 2000000000ULL) return fail(ctx, MJK_EUNSUPPORTED, "too many row-kernel tasks");
                 std::vector<int4> rt(tasks.size());
-                for (size_t t = 0; t < tasks.size(); ++t) rt[t] = tasks[t].second;
+                std::vector<int> rtp(tasks.size());
+                for (size_t t = 0; t < tasks.size(); ++t) {
+                    rt[t] = tasks[order[t]].second;
+                    rtp[t] = tprim[order[t]];
+                }
                 qc.nrt = (int)rt.size();
                 int rc;
-                if (qc.nrt > 0 && ((rc = upload(ctx, rt, &qc.rt)) || (rc = upload(ctx, rr, &qc.rr)))) return rc;
+                if (qc.nrt > 0 && ((rc = upload(ctx, rt, &qc.rt)) || (rc = upload(ctx, rr, &qc.rr)) ||
+                                   (rc = upload(ctx, rtp, &qc.rtp))))
+                    return rc;
             }
             b->qcls.push_back(qc);
         }
@@ -835,6 +856,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     if (const char* nc = getenv("MJK_NCOPY")) ctx->ncopy = std::max(1, std::min(16, atoi(nc)));
     if (const char* rc = getenv("MJK_ROW_CHUNK")) ctx->row_chunk = std::max(256, std::min(1 << 24, atoi(rc)));
     if (const char* rm = getenv("MJK_ROW_MIN_BLOCKS")) ctx->row_min_blocks = std::max(1, std::min(1 << 20, atoi(rm)));
+    if (const char* rq = getenv("MJK_ROW_PRIM")) ctx->row_prim = std::max(0, std::min(4096, atoi(rq)));
     if (const char* rb = getenv("MJK_ROW_BLOCKS")) ctx->row_blocks = std::max(256, std::min(1 << 20, atoi(rb)));
     if (const char* rl = getenv("MJK_ROW_LDS")) ctx->row_lds = (size_t)std::max(0, std::min(61440, atoi(rl)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
@@ -1036,6 +1058,8 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         if (shmem <= ctx->row_lds && qc.nrt > 0) {
             a.rt = qc.rt;
             a.rr = qc.rr;
+            a.rtp = qc.rtp;
+            a.rprim = qc.rprim;
             rr = launch_jk_rows(qc.la, qc.lb, qc.lc, qc.ld, qc.nrt, a, shmem, st);
             if (rr > 0) return fail(ctx, MJK_EDEVICE, "row kernel launch failed for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
         }
