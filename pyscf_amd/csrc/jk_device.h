@@ -1071,11 +1071,10 @@ void k_jk_row(const JKArgs a)
     c.pvalid = true;
     c.rA = uload(a.P.rA + p);
     c.AB = uload(a.P.rAB + p);
-    {
-        const int pbo = uload(a.rtp + blockIdx.x);   // this task's share of the bra's primitive pairs
-        c.pb0 = uload(a.P.prim0 + p) + pbo;
-        c.npb = min(a.rprim, uload(a.P.nprim + p) - pbo);
-    }
+    const int pbo = uload(a.rtp + blockIdx.x);   // this task's share of the bra's primitive pairs
+    c.pb0 = uload(a.P.prim0 + p) + pbo;
+    c.npb = min(a.rprim, uload(a.P.nprim + p) - pbo);
+    const unsigned long long quartet_tally = pbo == 0;   // a quartet counts once, in its first share
     const bool diagA = uload(a.P.shA + p) == uload(a.P.shB + p);
     const int oa = uload(a.P.cartA + p), ob = uload(a.P.cartB + p);
     const int cp = xcc_copy(a.ncopy);
@@ -1127,7 +1126,7 @@ void k_jk_row(const JKArgs a)
         if (!__ballot(c.alive)) continue;   // wave-uniform
         const bool use32 = a.fp32_tol > 0 && __ballot(c.alive && !small) == 0;
         if (c.alive) {
-            cnt[use32 ? 2 : 0] += 1;
+            cnt[use32 ? 2 : 0] += quartet_tally;
             cnt[use32 ? 3 : 1] += (unsigned long long)(c.npb * c.npk);
         }
         if (use32) row_digest<LA, LB, LC, LD, SPLIT, S, float>(a, c, kr, jr, krow, base, ld, oa, ob, cp);

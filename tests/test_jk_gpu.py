@@ -178,3 +178,20 @@ def test_row_and_tile_kernels_agree(oracle, row_lds):
         assert abs(k1 - k0).max() < 1e-11 * max(1, abs(k0).max())
     j0, k0 = oracle.get_jk(mol, dms, hermi=0, vhfopt=oracle.VHFOpt(mol, 1e-13))
     assert abs(got[1][0] - j0).max() < TOL * 10 and abs(got[1][1] - k0).max() < TOL * 10
+
+
+def test_quartet_count_matches_oracle(oracle):
+    # segmented basis (no general contractions): the engine evaluates exactly the
+    # shell quartets the reference's prescreen keeps, each counted once however
+    # its primitive pairs are shared out among row tasks
+    mol = gto.M(atom='''O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587
+                        O 3.1 0.2 0; H 3.1 -0.557 0.587; H 3.1 0.957 0.587
+                        O 0 2.9 0.3; H 0 2.143 0.887; H 0 3.657 0.887''', basis='def2-svp')
+    nao = mol.nao_nr()
+    c = numpy.random.default_rng(21).standard_normal((nao, 15))
+    dm = c @ c.T / 15
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    opt.basis.get_jk(dm[None], 1, True, True, 1e-13, stats=True)
+    nq = opt.basis.last_stats['quartets']
+    _, _, nq0 = oracle.direct_full(mol, dm[None], vhfopt=oracle.VHFOpt(mol, 1e-13), pair_stride=0)
+    assert abs(nq - nq0) <= max(2, 1e-4 * nq0), (nq, nq0)
