@@ -324,6 +324,8 @@ def main():
         k = L.mjk_profile_read(ctx, basis.handle, maxc, ptr(cls), ptr(ms), ptr(nq), ptr(npq), ptr(fl), ptr(nl))
         kinds = numpy.zeros(maxc, dtype=numpy.int32)
         L.mjk_profile_kinds(ctx, basis.handle, maxc, ptr(kinds))
+        span, nspan = ctypes.c_double(), ctypes.c_longlong()
+        eng.check(L.mjk_profile_span(ctx, basis.handle, ctypes.byref(span), ctypes.byref(nspan)), 'profile_span')
         L.mjk_profile(ctx, basis.handle, 0)
         classes = []
         for i in range(k):
@@ -338,7 +340,8 @@ def main():
         vk = numpy.empty_like(dm)
         eng.check(L.mjk_memcpy_d2h(ctx, ptr(vj), d_vj, nbytes), 'd2h')
         eng.check(L.mjk_memcpy_d2h(ctx, ptr(vk), d_vk, nbytes), 'd2h')
-        return {'t_total': t_total, 'lat': lat, 'classes': classes, 'stats': st, 'vj': vj, 'vk': vk}
+        return {'t_total': t_total, 'lat': lat, 'classes': classes, 'stats': st, 'vj': vj, 'vk': vk,
+                'span_ms': span.value / max(nspan.value, 1)}
 
     r64 = timed(0.0)
     rmx = timed(args.fp32_tol) if args.fp32_tol > 0 else None
@@ -351,7 +354,11 @@ def main():
         mfm.conv_tol = 1e-10
         emx = mfm.kernel()
     t_total, lat, classes, st = r64['t_total'], r64['lat'], r64['classes'], r64['stats']
-    dom = classes[0]
+    # the classes run concurrently on several streams: the dominant kernel is the
+    # one with the most algorithmic FLOPs; its duration is measured while it
+    # shares the GPU with the others
+    dom = max(classes, key=lambda c: c['flops'])
+    span_ms = r64['span_ms']
     ms_step = t_total / args.steps * 1e3
     builds = args.steps / t_total
     kernel_ms = sum(c['ms_per_launch'] for c in classes)
@@ -376,13 +383,18 @@ def main():
         'p50_ms': float(numpy.percentile(lat, 50) * 1e3),
         'p99_ms': float(numpy.percentile(lat, 99) * 1e3),
         'scf': scf_info,
-        'gpu_kernel_ms_per_build': kernel_ms,
-        'fp64_tflops_all_classes': total_flops / (kernel_ms * 1e-3) / 1e12,
+        'gpu_quartet_phase_ms_per_build': span_ms,
+        'sum_of_class_kernel_ms_per_build': kernel_ms,
+        'fp64_tflops_all_classes': total_flops / (span_ms * 1e-3) / 1e12,
         'roofline': {'bound': 'fp64', 'kernel': dom['kernel'], 'class': dom['class'],
                      'achieved': dom['tflops'], 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
                      'frac': dom['tflops'] * 1e12 / FP64_PEAK,
                      'traffic': traffic_bytes(dom['kernel']) if args.workload == 'c2' else None,
-                     'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch']},
+                     'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch'],
+                     'build': {'what': 'all quartet-class kernels of one J/K build, concurrent on 4 streams',
+                               'achieved': total_flops / (span_ms * 1e-3) / 1e12,
+                               'frac': total_flops / (span_ms * 1e-3) / FP64_PEAK,
+                               'flops': total_flops, 'ms': span_ms}},
         'classes': classes[:8],
     }
     if rmx is not None:

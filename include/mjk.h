@@ -123,7 +123,9 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n
                       int with_j, int with_k, double direct_scf_tol, double fp32_tol,
                       double* d_vj, double* d_vk, mjk_stats* stats);
 
-/* Per-quartet-class kernel profiling with HIP events on the launch stream.
+/* Per-quartet-class kernel profiling with HIP events on the launch stream
+ * (the classes of a build run concurrently on up to four streams, so the
+ * per-class times of one build overlap).
  * mjk_profile(…, 1) resets and starts accumulating times and work tallies
  * (each J/K build then synchronises at its end); mjk_profile(…, 2) does the
  * same with times only (the kernels then keep no tallies, as in an
@@ -139,6 +141,10 @@ int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* basis, int max_classes, int*
  * 0 tile kernel (k_jk), 1 row kernel (k_jk_row), 2 cooperative kernel
  * (k_jk_coop).  Returns the number of classes written. */
 int mjk_profile_kinds(mjk_ctx* ctx, const mjk_basis* basis, int max_classes, int* kind);
+/* Accumulated GPU time of the quartet phase of the profiled builds (from the
+ * end of the density transform to the end of the last class kernel, over all
+ * the context's streams) and the number of builds. */
+int mjk_profile_span(mjk_ctx* ctx, const mjk_basis* basis, double* ms, long long* builds);
 
 /* Device memory helpers (for callers without another GPU runtime). */
 int mjk_device_alloc(mjk_ctx* ctx, size_t bytes, void** ptr);

@@ -90,6 +90,7 @@ struct QClass {
     int X, Y, la, lb, lc, ld;
     int nbt = 0;
     long long nblocks = 0;
+    double work = 0;          // rough cost (ket tiles x components x roots), to spread classes over streams
     int4* bt = nullptr;
     int* bt_mol = nullptr;
     int* bt_off = nullptr;
@@ -142,6 +143,13 @@ struct mjk_ctx {
     std::map<void*, size_t> allocs;
     std::recursive_mutex mu;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // extra streams (default priority) on which the quartet classes of a build
+    // run concurrently; they fork from and join back into `stream`
+    static constexpr int NAUX = 3;
+    int nstreams = 4;                     // 1 + aux streams used, MJK_STREAMS
+    hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[NAUX] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -243,6 +251,8 @@ struct mjk_basis {
     std::vector<double> prof_ms;
     std::vector<long long> prof_nq, prof_np, prof_launches;
     std::vector<int> prof_kind;           // kernel of the class's last launch: 0 tile, 1 row, 2 cooperative
+    double prof_span_ms = 0;              // quartet phase of the profiled builds (all classes, all streams)
+    long long prof_builds = 0;
 };
 
 namespace {
@@ -318,6 +328,8 @@ int build_tiles(mjk_basis* b, double cutoff)
             }
             qc.nbt = (int)bt.size();
             qc.nblocks = total;
+            qc.work = (double)total * ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) * ncart(qc.ld) *
+                      ((qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1);
             if (qc.nbt > 0) {
                 int rc;
                 if ((rc = upload(ctx, bt, &qc.bt))) return rc;
@@ -859,7 +871,16 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     if (const char* rq = getenv("MJK_ROW_PRIM")) ctx->row_prim = std::max(0, std::min(4096, atoi(rq)));
     if (const char* rb = getenv("MJK_ROW_BLOCKS")) ctx->row_blocks = std::max(256, std::min(1 << 20, atoi(rb)));
     if (const char* rl = getenv("MJK_ROW_LDS")) ctx->row_lds = (size_t)std::max(0, std::min(61440, atoi(rl)));
+    if (const char* ns = getenv("MJK_STREAMS")) ctx->nstreams = std::max(1, std::min(1 + mjk_ctx::NAUX, atoi(ns)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
+    bool streams_ok = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < mjk_ctx::NAUX; ++i)
+        streams_ok = streams_ok && hipStreamCreateWithFlags(&ctx->aux[i], hipStreamNonBlocking) == hipSuccess &&
+                     hipEventCreateWithFlags(&ctx->ev_join[i], hipEventDisableTiming) == hipSuccess;
+    if (!streams_ok) {
+        mjk_ctx_destroy(ctx);
+        return MJK_EDEVICE;
+    }
     std::vector<double> tab(RYS_BUF_SIZE);
     rys_fill_buffer(tab.data());
     if (upload(ctx, tab, &ctx->rys) || upload_c2s(ctx->stream) || hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -879,6 +900,14 @@ void mjk_ctx_destroy(mjk_ctx* ctx)
     ctx->allocs.clear();
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < mjk_ctx::NAUX; ++i) {
+        if (ctx->aux[i]) {
+            (void)hipStreamSynchronize(ctx->aux[i]);
+            (void)hipStreamDestroy(ctx->aux[i]);
+        }
+        if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
+    }
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1041,9 +1070,25 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.rys = ctx->rys;
     a.pmol = b->d_pmol;
     a.nbucket = NBUCKET;
-    for (size_t c = 0; c < b->qcls.size(); ++c) {
+    // classes, heaviest first, each on the least loaded of the streams (they all
+    // add into the same J'/K' with atomics, so their order does not matter)
+    const int nst = ctx->nstreams;
+    hipStream_t streams[1 + mjk_ctx::NAUX] = {st, ctx->aux[0], ctx->aux[1], ctx->aux[2]};
+    if (nst > 1) {
+        HIPCHK(ctx, hipEventRecord(ctx->ev_fork, st));
+        for (int i = 1; i < nst; ++i) HIPCHK(ctx, hipStreamWaitEvent(streams[i], ctx->ev_fork, 0));
+    }
+    std::vector<size_t> corder(b->qcls.size());
+    std::iota(corder.begin(), corder.end(), 0);
+    std::stable_sort(corder.begin(), corder.end(),
+                     [&](size_t u, size_t v) { return b->qcls[u].work > b->qcls[v].work; });
+    double load[1 + mjk_ctx::NAUX] = {0, 0, 0, 0};
+    for (size_t c : corder) {
         const QClass& qc = b->qcls[c];
         if (qc.nblocks == 0) continue;
+        const int si = (int)(std::min_element(load, load + nst) - load);
+        load[si] += qc.work;
+        hipStream_t st = streams[si];
         if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c], st));
         a.bt = qc.bt;
         a.bt_mol = qc.bt_mol;
@@ -1068,6 +1113,10 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         if (prof) b->prof_kind[c] = rr == 0 ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
         if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c + 1], st));
     }
+    for (int i = 1; i < nst; ++i) {
+        HIPCHK(ctx, hipEventRecord(ctx->ev_join[i - 1], streams[i]));
+        HIPCHK(ctx, hipStreamWaitEvent(st, ctx->ev_join[i - 1], 0));
+    }
     HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
     if (launch_cart2sph(xform(b), b->maxnao, b->J, b->K, b->slot_c, n_dm, hermi, with_j, with_k, d_vj, d_vk,
                         b->slot_s, ncopy, jcopy, kcopy, st))
@@ -1075,6 +1124,10 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
     if (prof) {
         HIPCHK(ctx, hipStreamSynchronize(st));
+        float span = 0;
+        HIPCHK(ctx, hipEventElapsedTime(&span, ctx->ev[1], ctx->ev[2]));
+        b->prof_span_ms += span;
+        b->prof_builds += 1;
         std::vector<unsigned long long> cnt(4 * NQCLASS_MAX, 0);
         if (tally)
             HIPCHK(ctx, hipMemcpy(cnt.data(), b->counters, sizeof(unsigned long long) * cnt.size(),
@@ -1164,6 +1217,8 @@ int mjk_profile(mjk_ctx* ctx, mjk_basis* b, int enable)
     b->prof_np.assign(n, 0);
     b->prof_launches.assign(n, 0);
     b->prof_kind.assign(n, -1);
+    b->prof_span_ms = 0;
+    b->prof_builds = 0;
     if (enable < 0 || enable > 2) return fail(ctx, MJK_EINVAL, "profile mode must be 0, 1 or 2");
     b->profiling = enable != 0;
     b->prof_counts = enable == 1;
@@ -1191,6 +1246,14 @@ int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* b, int max_classes, int* cls
         ++k;
     }
     return k;
+}
+
+int mjk_profile_span(mjk_ctx* ctx, const mjk_basis* b, double* ms, long long* builds)
+{
+    if (!ctx || !b || !ms || !builds) return MJK_EINVAL;
+    *ms = b->prof_span_ms;
+    *builds = b->prof_builds;
+    return MJK_OK;
 }
 
 int mjk_profile_kinds(mjk_ctx* ctx, const mjk_basis* b, int max_classes, int* kind)

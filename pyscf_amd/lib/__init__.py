@@ -63,6 +63,7 @@ def _declare(lib):
     lib.mjk_profile.argtypes = [c_void_p, c_void_p, c_int]
     lib.mjk_profile_read.argtypes = [c_void_p, c_void_p, c_int] + [c_void_p] * 6
     lib.mjk_profile_kinds.argtypes = [c_void_p, c_void_p, c_int, c_void_p]
+    lib.mjk_profile_span.argtypes = [c_void_p, c_void_p, P(c_double), P(ctypes.c_longlong)]
     for name in ('mjk_int1e',):
         if hasattr(lib, name):
             getattr(lib, name).argtypes = [c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]
