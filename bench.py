@@ -40,16 +40,19 @@ def progress(msg):
 _T0 = time.perf_counter()
 
 
-def kernel_name(cls, kind):
+def kernel_name(cls, kind, rsh=False):
     """Demangled name (as rocprofv3 prints it, less the argument list) of the
     kernel a quartet class ran on: 0 tile, 1 row, 2 cooperative; classes cut
-    into slices run one kernel per slice and are named by slice 0."""
+    into slices run one kernel per slice and are named by slice 0.  rsh: the
+    row-kernel variant of the range-separated operators."""
     from pyscf_amd.build import jk_split
     la, lb, lc, ld = cls
     if kind == 2:
         return 'mjk::k_jk_coop<%d, %d, %d, %d>' % cls
-    base = 'mjk::k_jk_row' if kind == 1 else 'mjk::k_jk'
-    return '%s<%d, %d, %d, %d, %d, 0>' % (base, la, lb, lc, ld, jk_split(la, lb, lc, ld))
+    if kind == 1:
+        return 'mjk::k_jk_row<%d, %d, %d, %d, %d, 0, %s>' % (la, lb, lc, ld, jk_split(la, lb, lc, ld),
+                                                             'true' if rsh else 'false')
+    return 'mjk::k_jk<%d, %d, %d, %d, %d, 0>' % (la, lb, lc, ld, jk_split(la, lb, lc, ld))
 
 
 def traffic_bytes(kernel):
@@ -82,6 +85,9 @@ def parse():
                          '(profiling runs: every J/K launch is then one of the converged density)')
     ap.add_argument('--cpu-seconds', type=float, default=10.0)
     ap.add_argument('--fp32-tol', type=float, default=1e-7)
+    ap.add_argument('--omega', type=float, default=0.0,
+                    help='c1-c3: time J/K builds of the range-separated operator (> 0 erf, < 0 erfc) '
+                         'on the converged Coulomb density')
     return ap.parse_args()
 
 
@@ -269,12 +275,16 @@ def main():
         mf.conv_tol = 1e-10
         e64 = mf.kernel()
         dm = numpy.ascontiguousarray(mf.dm)
-        opt = mf._opt
+        opt = mf._opt[None]
         scf_info = {'e_tot': e64, 'cycles': mf.cycles, 'converged': mf.converged,
                     'seconds': time.perf_counter() - t0}
         if args.dm_out:
             numpy.save(args.dm_out, dm)
 
+    if args.omega:
+        # bounds and builds of the range-separated operator, on the same density
+        with mol.with_range_coulomb(args.omega):
+            opt = _vhf.VHFOpt(mol, direct_scf_tol=tol)
     eng = opt.engine
     L = eng.lib
     basis = opt.basis
@@ -332,7 +342,8 @@ def main():
             key = tuple(int(x) for x in cls[4 * i:4 * i + 4])
             nq1, fl1 = work.get(key, (0, 0.0))
             t1 = ms[i] / nl[i]
-            classes.append({'class': '(%d%d|%d%d)' % key, 'kernel': kernel_name(key, int(kinds[i])),
+            kname = kernel_name(key, int(kinds[i]), bool(args.omega))
+            classes.append({'class': '(%d%d|%d%d)' % key, 'kernel': kname,
                             'ms_per_launch': t1, 'quartets': nq1, 'flops': fl1,
                             'tflops': fl1 / (t1 * 1e-3) / 1e12 if t1 > 0 else 0.0})
         classes.sort(key=lambda c: -c['ms_per_launch'])
@@ -349,7 +360,7 @@ def main():
         L.mjk_device_free(ctx, p)
     # mixed-precision SCF energy
     emx = None
-    if rmx is not None and e64 is not None:
+    if rmx is not None and e64 is not None and not args.omega:
         mfm = rhf.RHF(mol, direct_scf_tol=tol, fp32_tol=args.fp32_tol)
         mfm.conv_tol = 1e-10
         emx = mfm.kernel()
@@ -377,7 +388,7 @@ def main():
         'dtype': 'f64',
         'data': 'synthetic geometry; converged RHF density from this engine',
         'config': {'workload': desc + ', one J/K build of the converged RHF density, hermi=1, '
-                   'direct_scf_tol=1e-13', 'nao': nao, 'n_dm': 1, 'precision': 'fp64'},
+                   'direct_scf_tol=1e-13', 'nao': nao, 'n_dm': 1, 'precision': 'fp64', 'omega': args.omega},
         'quartets_per_s': st.quartets * builds,
         'quartets_per_build': int(st.quartets),
         'p50_ms': float(numpy.percentile(lat, 50) * 1e3),
@@ -413,7 +424,8 @@ def main():
             'dE_vs_fp64': emx - e64 if emx is not None else None,
         }
     if not args.no_cpu:
-        out['cpu_baseline'] = cpu_baseline(mol, dm, tol, args.cpu_seconds)
+        with mol.with_range_coulomb(args.omega or None):
+            out['cpu_baseline'] = cpu_baseline(mol, dm, tol, args.cpu_seconds)
     print(json.dumps(out))
 
 

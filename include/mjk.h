@@ -84,6 +84,11 @@ int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes);
 /* Upload one molecule's basis and build its shell pairs, Schwarz bounds
  * (the q_cond of CVHFnr_int2e_q_cond, pyscf/lib/vhf/optimizer.c:408-454)
  * and quartet tile lists for screening at `cutoff` (<= 0: none).
+ * The two-electron operator is env[8] (PTR_RANGE_OMEGA, as libcint reads it;
+ * set by Mole.with_range_coulomb, pyscf/gto/mole.py:2930-2979): 0 the Coulomb
+ * 1/r, > 0 erf(omega r)/r, < 0 erfc(|omega| r)/r.  It holds for the bounds
+ * and for every J/K build of the handle (one handle per omega, like the
+ * per-omega optimizers of pyscf/scf/hf.py:2062-2067).
  * Replaces _VHFOpt.__init__/init_cvhf_direct (pyscf/scf/_vhf.py:151-206). */
 int mjk_basis_upload(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas,
                      const double* env, double cutoff, mjk_basis** out);

@@ -44,6 +44,8 @@
 #define PTR_EXP 5
 #define PTR_COEFF 6
 #define ATOM_OF 0
+#define PTR_RANGE_OMEGA 8
+#define ORC_MAXQ (ORC_NGL * 64)
 
 static double gl_t[ORC_NGL], gl_w[ORC_NGL];
 static double herm_x[ORC_MAXROOTS + 1][ORC_MAXROOTS];
@@ -225,6 +227,66 @@ void orc_rys_roots(int n, double T, double *x, double *w)
         sort_pairs(n, x, w);
 }
 
+/* ---------------- range-separated Coulomb operators ---------------- */
+/* Quadrature points (x = t^2, w) of the t-integral of one primitive quartet
+ * for the operator selected by omega = env[PTR_RANGE_OMEGA]
+ * (pyscf/gto/mole.py:3055-3069 with_range_coulomb; libcint reads the slot):
+ *   omega = 0: 1/r,            t in [0, 1]
+ *   omega > 0: erf(omega r)/r,  t in [0, sqrt(theta)]
+ *   omega < 0: erfc(|omega| r)/r, t in [sqrt(theta), 1]
+ * with theta = omega^2 / (omega^2 + rho), rho = pq / (p + q): the erf kernel
+ * is (2/sqrt(pi)) int_0^omega exp(-s^2 r^2) ds, and the Rys variable
+ * t^2 = s^2 / (s^2 + rho) maps s = omega to t^2 = theta.
+ *
+ * Mode 0 (default), Rys quadrature with n = L/2 + 1 points per set: for
+ * erf, t = sqrt(theta) s turns the weight into exp(-theta T s^2) on [0, 1],
+ * so x_i = theta x_i(theta T), w_i = sqrt(theta) w_i(theta T); erfc is the
+ * full-range set plus the erf set with negated weights (erfc = 1 - erf).
+ * Mode 1 (a check of mode 0): composite Gauss-Legendre in t over the
+ * operator's own interval, orc_panels panels of ORC_NGL points, which uses
+ * neither the Rys roots, nor the theta scaling, nor the erfc split. */
+static int orc_quad_mode = 0, orc_panels = 8;
+
+void orc_set_quadrature(int mode, int panels)
+{
+        orc_quad_mode = mode;
+        orc_panels = panels < 1 ? 1 : (panels > ORC_MAXQ / ORC_NGL ? ORC_MAXQ / ORC_NGL : panels);
+}
+
+static int quad_points(int n, double T, double rho, double omega, double *x, double *w)
+{
+        double theta = 1.;
+        if (omega != 0.) theta = omega * omega / (omega * omega + rho);
+        if (orc_quad_mode == 1) {
+                double t0 = omega < 0. ? sqrt(theta) : 0., t1 = omega > 0. ? sqrt(theta) : 1.;
+                double h = (t1 - t0) / orc_panels;
+                int k, i, m = 0;
+                for (k = 0; k < orc_panels; k++)
+                        for (i = 0; i < ORC_NGL; i++, m++) {
+                                double t = t0 + h * (k + gl_t[i]);
+                                x[m] = t * t;
+                                w[m] = h * gl_w[i] * exp(-T * t * t);
+                        }
+                return m;
+        }
+        if (omega == 0.) {
+                orc_rys_roots(n, T, x, w);
+                return n;
+        }
+        int off = 0, i;
+        if (omega < 0.) {
+                orc_rys_roots(n, T, x, w);
+                off = n;
+        }
+        orc_rys_roots(n, theta * T, x + off, w + off);
+        double sw = omega > 0. ? sqrt(theta) : -sqrt(theta);
+        for (i = 0; i < n; i++) {
+                x[off + i] *= theta;
+                w[off + i] *= sw;
+        }
+        return off + n;
+}
+
 /* ---------------- Cartesian -> real solid harmonics ---------------- */
 /* Cartesian order per shell: for lx = l..0, for ly = l-lx..0 (libcint order:
  * d = xx,xy,xz,yy,yz,zz).  Spherical order m = -l..l, except p = x,y,z. */
@@ -317,7 +379,8 @@ static void get_shell(Shell *s, int ish, const int *atm, const int *bas, const d
 /* Cartesian contracted block for one shell quartet.
  * cart[((((ci*Nj + cj)*Nk + ck)*Nl + cl)] with Ni = nctr_i*ncart_i ordered
  * (ctr, cart). */
-static void eri_cart(double *cart, const Shell *si, const Shell *sj, const Shell *sk, const Shell *sl)
+static void eri_cart(double *cart, const Shell *si, const Shell *sj, const Shell *sk, const Shell *sl,
+                     double omega)
 {
         int li = si->l, lj = sj->l, lk = sk->l, ll = sl->l;
         int ni = ncart(li), nj = ncart(lj), nk = ncart(lk), nl = ncart(ll);
@@ -349,7 +412,8 @@ static void eri_cart(double *cart, const Shell *si, const Shell *sj, const Shell
         int nprim_q = ni * nj * nk * nl;
         double *gp = malloc(sizeof(double) * nprim_q);
         double g[3][G1][G1];
-        double I[ORC_MAXROOTS][3][H1][H1][H1][H1];
+        double I[3][H1][H1][H1][H1];
+        double *xr = malloc(sizeof(double) * ORC_MAXQ), *wr = malloc(sizeof(double) * ORC_MAXQ);
         int pi, pj, pk, pl;
         for (pi = 0; pi < si->nprim; pi++)
         for (pj = 0; pj < sj->nprim; pj++) {
@@ -372,9 +436,10 @@ static void eri_cart(double *cart, const Shell *si, const Shell *sj, const Shell
                         double rho = p * q / pq;
                         double T = rho * PQ2;
                         double fac = 2. * pow(M_PI, 2.5) / (p * q * sqrt(pq)) * Kab * Kcd;
-                        double xr[ORC_MAXROOTS], wr[ORC_MAXROOTS];
-                        orc_rys_roots(nroots, T, xr, wr);
-                        for (r = 0; r < nroots; r++) {
+                        int nq = quad_points(nroots, T, rho, omega, xr, wr);
+                        int fi, fj, fk, fl, n = 0;
+                        for (n = 0; n < nprim_q; n++) gp[n] = 0;
+                        for (r = 0; r < nq; r++) {
                                 double t2 = xr[r];
                                 double B00 = 0.5 * t2 / pq;
                                 double B10 = 0.5 / p * (1. - t2 * q / pq);
@@ -412,21 +477,17 @@ static void eri_cart(double *cart, const Shell *si, const Shell *sj, const Shell
                                                                 hk[c][e] = hk[c + 1][e - 1] + CD[d] * hk[c][e - 1];
                                                 for (c = 0; c <= lk; c++)
                                                         for (e = 0; e <= ll; e++)
-                                                                I[r][d][a][b][c][e] = hk[c][e];
+                                                                I[d][a][b][c][e] = hk[c][e];
                                         }
                                 }
-                        }
-                        int fi, fj, fk, fl, n = 0;
-                        for (fi = 0; fi < ni; fi++)
-                        for (fj = 0; fj < nj; fj++)
-                        for (fk = 0; fk < nk; fk++)
-                        for (fl = 0; fl < nl; fl++, n++) {
-                                double s = 0;
-                                for (r = 0; r < nroots; r++)
-                                        s += I[r][0][ix[0][fi]][ix[1][fj]][ix[2][fk]][ix[3][fl]]
-                                           * I[r][1][iy[0][fi]][iy[1][fj]][iy[2][fk]][iy[3][fl]]
-                                           * I[r][2][iz[0][fi]][iz[1][fj]][iz[2][fk]][iz[3][fl]];
-                                gp[n] = s;
+                                n = 0;
+                                for (fi = 0; fi < ni; fi++)
+                                for (fj = 0; fj < nj; fj++)
+                                for (fk = 0; fk < nk; fk++)
+                                for (fl = 0; fl < nl; fl++, n++)
+                                        gp[n] += I[0][ix[0][fi]][ix[1][fj]][ix[2][fk]][ix[3][fl]]
+                                               * I[1][iy[0][fi]][iy[1][fj]][iy[2][fk]][iy[3][fl]]
+                                               * I[2][iz[0][fi]][iz[1][fj]][iz[2][fk]][iz[3][fl]];
                         }
                         int ci, cj, ck, cl;
                         for (ci = 0; ci < si->nctr; ci++)
@@ -448,11 +509,13 @@ static void eri_cart(double *cart, const Shell *si, const Shell *sj, const Shell
                 }
         }
         free(gp);
+        free(xr);
+        free(wr);
 }
 
 /* Spherical block in Fortran order out[i + di*(j + dj*(k + dk*l))],
  * d = (2l+1)*nctr, AO order (ctr, m). */
-static void eri_sph(double *out, const Shell *sh[4])
+static void eri_sph(double *out, const Shell *sh[4], double omega)
 {
         int Nc[4], Ns[4], a;
         for (a = 0; a < 4; a++) {
@@ -461,7 +524,7 @@ static void eri_sph(double *out, const Shell *sh[4])
         }
         size_t ncart_tot = (size_t)Nc[0] * Nc[1] * Nc[2] * Nc[3];
         double *cart = malloc(sizeof(double) * ncart_tot);
-        eri_cart(cart, sh[0], sh[1], sh[2], sh[3]);
+        eri_cart(cart, sh[0], sh[1], sh[2], sh[3], omega);
         /* transform index by index; layout kept C-order (i slowest) */
         int dims[4] = {Nc[0], Nc[1], Nc[2], Nc[3]};
         double *cur = cart;
@@ -517,7 +580,7 @@ int orc_int2e_sph(double *out, const int *shls, const int *atm, int natm,
                 get_shell(&s[a], shls[a], atm, bas, env);
                 sp[a] = &s[a];
         }
-        eri_sph(out, sp);
+        eri_sph(out, sp, env[PTR_RANGE_OMEGA]);
         return 1;
 }
 

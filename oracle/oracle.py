@@ -16,6 +16,12 @@ Python side of the oracle:
                      (pyscf/scf/hf.py:2416-2427); core-Hamiltonian initial guess
   * ``boys_mp``      Boys function F_m(T) to 30 digits with mpmath (pins the
                      Rys roots: sum_i w_i x_i^m == F_m(T))
+  * ``set_quadrature`` the t-integral of every primitive quartet by Rys
+                     roots (mode 0) or by composite Gauss-Legendre over the
+                     operator's interval (mode 1, a check of mode 0 that
+                     shares none of its root machinery).  The operator is
+                     env[PTR_RANGE_OMEGA] (8), as in libcint: 0 Coulomb,
+                     > 0 erf(omega r)/r, < 0 erfc(|omega| r)/r.
 The ERI engine (Rys quadrature, Stieltjes/Golub-Welsch roots) and the s8
 shell-quartet loop are in ``jk_oracle.c`` beside this file.
 """
@@ -81,6 +87,10 @@ def hermi_triu(mat, hermi=1):
     elif hermi == 2:
         mat[..., iu[0], iu[1]] = -mat[..., iu[1], iu[0]]
     return mat
+
+
+def set_quadrature(mode, panels=8):
+    lib().orc_set_quadrature(ctypes.c_int(mode), ctypes.c_int(panels))
 
 
 def rys_roots(n, T):

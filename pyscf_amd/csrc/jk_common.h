@@ -81,6 +81,7 @@ struct JKArgs {
     int screen;
     double fp32_tol;        // > 0: waves whose quartets all have q_ij q_kl dmax < fp32_tol run in FP32
     const double* rys;      // Rys table buffer
+    double omega;           // operator: 0 Coulomb, > 0 erf(omega r)/r, < 0 erfc(|omega| r)/r
     unsigned long long* counters;  // {FP64 quartets, FP64 primitive quartets, FP32 quartets, FP32 primitive quartets}
     // row kernel: block b takes bra pair rt[b].x and the kets rt[b].y .. min(rt[b].z, n) - 1 of
     // the n surviving kets of the ket class, i.e. of the concatenated ket ranges
@@ -100,6 +101,7 @@ struct QCondArgs {
     int p0, np;             // pair range of one class
     double* qout;           // Schwarz bound per pair
     const double* rys;
+    double omega;           // operator, as in JKArgs
 };
 
 struct XformArgs {

@@ -308,7 +308,8 @@ __device__ __forceinline__ void add_root(const PrimQuartet& m, double rx, double
 
 // Contracted Cartesian block of one quartet restricted to slice S, times
 // the quartet's degeneracy factor, in FP64 (recurrences in precision R).
-template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R, bool UNIFORM_BRA = false>
+// RSH: the kernel also evaluates the range-separated operators (a.omega != 0).
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R, bool UNIFORM_BRA = false, bool RSH = false>
 __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
                                           double (&gd)[SliceRange<LA, LB, LC, LD, SPLIT, S>::NFL])
 {
@@ -323,6 +324,26 @@ __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
     for (int ib = 0; ib < c.npb; ++ib) {
         const double4 Pp = UNIFORM_BRA ? uload(a.P.ppP + c.pb0 + ib) : a.P.ppP[c.pb0 + ib];
         const double2 Kp = UNIFORM_BRA ? uload(a.P.ppK + c.pb0 + ib) : a.P.ppK[c.pb0 + ib];
+        if (RSH && a.omega != 0.0) {   // range-separated operator, one ket primitive at a time
+            for (int ik = 0; ik < c.npk; ++ik) {
+                double T;
+                const int t0 = c.tq + ik * c.ts;
+                const PrimQuartet m = prim_quartet(Pp, Kp, a.P.tP[t0], a.P.tK[t0], c, T);
+                const double rho = m.p * m.q * m.ipq;
+                for (int pass = 0; pass < rsh_passes(a.omega); ++pass) {
+                    double Tp = T, sx, sw;
+                    rsh_scale(a.omega, rho, pass, Tp, sx, sw);
+                    const RysCursor<NR> rys(a.rys, Tp);
+#pragma unroll 1
+                    for (int r = 0; r < NR; ++r) {
+                        double rx, rw;
+                        rys.root(r, rx, rw);
+                        add_root<LA, LB, LC, LD, SPLIT, S, R>(m, sx * rx, sw * rw, c, gout);
+                    }
+                }
+            }
+            continue;
+        }
         int ik = 0;
         if (NFL <= MJK_PAIR_NFL) {
             for (; ik + 1 < c.npk; ik += 2) {
@@ -370,7 +391,7 @@ __device__ __forceinline__ void quartet_slice(const JKArgs& a, const QuartetCtx&
     constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
     constexpr int MA = SR::MA, MB = SR::MB, MC = SR::MC;
     double gd[SR::NFL];
-    eri_slice<LA, LB, LC, LD, SPLIT, S, R>(a, c, gd);
+    eri_slice<LA, LB, LC, LD, SPLIT, S, R, false, true>(a, c, gd);
 
     const long long base = a.M.dbase[c.mol];
     const int ld = a.M.ld[c.mol];
@@ -709,14 +730,18 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
 
         constexpr int PB = Sh::PB;
         const int npq = npb * npk;
-        for (int j0 = 0; j0 < npq; j0 += PB) {      // PB primitive quartets per phase
-            const int nb = min(PB, npq - j0);
+        // erfc operator: every primitive quartet twice (two root sets)
+        const int nvq = rsh_passes(a.omega) * npq;
+        for (int j0 = 0; j0 < nvq; j0 += PB) {      // PB primitive quartets per phase
+            const int nb = min(PB, nvq - j0);
             {
                 for (int it = tid; it < nb * NITEM; it += NT) {
                     const int j = it / NITEM;
                     const int r = (it - j * NITEM) / 3;
                     const int d = it - j * NITEM - 3 * r;
-                    const int ib = (j0 + j) / npk, ik = j0 + j - ib * npk;
+                    const int pass = j0 + j >= npq;
+                    const int jq = j0 + j - pass * npq;
+                    const int ib = jq / npk, ik = jq - ib * npk;
                     const double4 Pp = a.P.ppP[pb0 + ib];
                     const double2 Kp = a.P.ppK[pb0 + ib];
                     const double4 Qp = a.P.ppP[pk0 + ik];
@@ -725,10 +750,14 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
                     const double sipq = rsqrt_nr(pq);
                     const double ipq = sipq * sipq;
                     const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
-                    const double T = Pp.w * Qp.w * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
+                    double T = Pp.w * Qp.w * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
+                    double sx, sw;
+                    rsh_scale(a.omega, Pp.w * Qp.w * ipq, pass, T, sx, sw);
                     const RysCursor<NR> rys(a.rys, T);
                     double rx, rw;
                     rys.root(r, rx, rw);
+                    rx *= sx;
+                    rw *= sw;
                     const double u = rx * ipq;
                     const double PQd = d == 0 ? PQx : d == 1 ? PQy : PQz;
                     const double PAd = d == 0 ? Pp.x - rA.x : d == 1 ? Pp.y - rA.y : Pp.z - rA.z;
@@ -889,7 +918,7 @@ __device__ __forceinline__ void lds_add(double* p, double v)
     atomicAdd(p, v);
 }
 
-template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R>
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R, bool RSH>
 __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c, double* kr, double* jr,
                                            int krow, long long base, int ld, int oa, int ob, int cp)
 {
@@ -898,7 +927,7 @@ __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c,
     constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
     constexpr int MA = SR::MA, MB = SR::MB, MC = SR::MC;
     double gd[SR::NFL];
-    eri_slice<LA, LB, LC, LD, SPLIT, S, R, true>(a, c, gd);
+    eri_slice<LA, LB, LC, LD, SPLIT, S, R, true, RSH>(a, c, gd);
     const int oc = c.qvalid ? a.P.cartA[c.q] : 0, od = c.qvalid ? a.P.cartB[c.q] : 0;
     const int lane = threadIdx.x & 63;
     for (int s = 0; s < a.n_j; ++s) {
@@ -1028,8 +1057,10 @@ struct RowOccupancy {
         (LA == 2 && LB == 1 && LC == 1 && LD == 0) || (LA == 2 && LB == 0 && LC == 1 && LD == 1) ? 2 : MJK_ROW_WPE;
 };
 
-template <int LA, int LB, int LC, int LD, int SPLIT, int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowOccupancy<LA, LB, LC, LD>::value)))
+// RSH: the variant for the range-separated operators (a.omega != 0); the
+// Coulomb variant keeps its registers free of that path.
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, bool RSH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSH ? 1 : RowOccupancy<LA, LB, LC, LD>::value)))
 void k_jk_row(const JKArgs a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB);
@@ -1129,8 +1160,8 @@ void k_jk_row(const JKArgs a)
             cnt[use32 ? 2 : 0] += quartet_tally;
             cnt[use32 ? 3 : 1] += (unsigned long long)(c.npb * c.npk);
         }
-        if (use32) row_digest<LA, LB, LC, LD, SPLIT, S, float>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
-        else row_digest<LA, LB, LC, LD, SPLIT, S, double>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
+        if (use32) row_digest<LA, LB, LC, LD, SPLIT, S, float, RSH>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
+        else row_digest<LA, LB, LC, LD, SPLIT, S, double, RSH>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
     }
     __syncthreads();
     // flush the accumulated rows of K' and the J'[a,b] block
@@ -1166,11 +1197,11 @@ void k_jk_row(const JKArgs a)
     }
 }
 
-template <int LA, int LB, int LC, int LD, int S>
+template <int LA, int LB, int LC, int LD, int S, bool RSH>
 int launch_jk_row(int nblocks, const JKArgs& a, size_t shmem, hipStream_t st)
 {
     constexpr int SPLIT = jk_split(LA, LB, LC, LD);
-    hipLaunchKernelGGL((k_jk_row<LA, LB, LC, LD, SPLIT, S>), dim3(nblocks), dim3(256), shmem, st, a);
+    hipLaunchKernelGGL((k_jk_row<LA, LB, LC, LD, SPLIT, S, RSH>), dim3(nblocks), dim3(256), shmem, st, a);
     return (int)hipGetLastError();
 }
 

@@ -35,11 +35,16 @@ __global__ __launch_bounds__(64) void k_qcond(const QCondArgs a)
             const double PQx = Pp.x - Qp.x, PQy = Pp.y - Qp.y, PQz = Pp.z - Qp.z;
             const double T = Pp.w * Qp.w * ipq * (PQx * PQx + PQy * PQy + PQz * PQz);
             const double pref = Kp.x * Kq.x * sqrt(ipq);
-            const RysCursor<NR> rys(a.rys, T);
+            for (int pass = 0; pass < rsh_passes(a.omega); ++pass) {   // range-separated operators
+            double Tp = T, sx, sw;
+            rsh_scale(a.omega, Pp.w * Qp.w * ipq, pass, Tp, sx, sw);
+            const RysCursor<NR> rys(a.rys, Tp);
 #pragma unroll 1
             for (int r = 0; r < NR; ++r) {
                 double rx, rw;
                 rys.root(r, rx, rw);
+                rx *= sx;
+                rw *= sw;
                 const double u = rx * ipq;
                 const double B00 = 0.5 * u;
                 const double B10 = 0.5 * Kp.y * (1.0 - u * Qp.w);
@@ -83,6 +88,7 @@ __global__ __launch_bounds__(64) void k_qcond(const QCondArgs a)
                     }
                     acc[ma][mb] += v;
                 }
+            }
             }
         }
     }

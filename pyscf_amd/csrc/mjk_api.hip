@@ -57,7 +57,7 @@ namespace {
 
 constexpr double EXPCUTOFF = 60.0;   // drop primitive pairs with exp(-ab/p AB^2) < e^-60
 constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8, PTR_COORD = 1, ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2,
-              NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6;
+              NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6, PTR_RANGE_OMEGA = 8;
 constexpr int NQCLASS_MAX = NPAIRCLASS * (NPAIRCLASS + 1) / 2;
 
 // Shell pairs of a class are grouped by their number of primitive pairs so
@@ -90,7 +90,8 @@ struct QClass {
     int X, Y, la, lb, lc, ld;
     int nbt = 0;
     long long nblocks = 0;
-    double work = 0;          // rough cost (ket tiles x components x roots), to spread classes over streams
+    double work = 0;          // rough cost (ket tiles x components x roots)
+    double est = 0;           // algorithmic FLOPs of the Schwarz-prescreened quartets (an upper bound)
     int4* bt = nullptr;
     int* bt_mol = nullptr;
     int* bt_off = nullptr;
@@ -147,6 +148,7 @@ struct mjk_ctx {
     // run concurrently; they fork from and join back into `stream`
     static constexpr int NAUX = 3;
     int nstreams = 4;                     // 1 + aux streams used, MJK_STREAMS
+    int sched = 1;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[NAUX] = {nullptr, nullptr, nullptr};
@@ -210,6 +212,7 @@ int upload(mjk_ctx* ctx, const std::vector<T>& v, T** d)
 struct mjk_basis {
     mjk_ctx* ctx = nullptr;
     int nmol = 0;
+    double omega = 0;   // two-electron operator, env[PTR_RANGE_OMEGA]: 0 1/r, > 0 erf, < 0 erfc
     std::vector<MolHost> mols;
     int maxld = 0, maxnao = 0, maxnsh = 0;
     long long slot_s = 0, slot_c = 0, cond_total = 0;
@@ -294,6 +297,7 @@ int build_tiles(mjk_basis* b, double cutoff)
             std::vector<int4> bt;
             std::vector<int> btm, off(1, 0);
             long long total = 0;
+            double est_q = 0, est_p = 0;   // Schwarz-prescreened quartets and primitive quartets
             for (int m = 0; m < nmol; ++m)
             for (int bx = 0; bx < NBUCKET; ++bx) {
                 const size_t sx = ((size_t)X * nmol + m) * NBUCKET + bx;
@@ -320,6 +324,13 @@ int build_tiles(mjk_basis* b, double cutoff)
                     if (nkt <= 0) continue;
                     bt.push_back(make_int4(p0, p1, y0, qend));
                     btm.push_back(m);
+                    for (int p = p0; p < p1; ++p) {
+                        const int qe = X == Y ? std::min(qend, p + 1) : qend;
+                        if (qe > y0) {
+                            est_q += qe - y0;
+                            est_p += (double)b->h_nprim[p] * (b->pcost[qe] - b->pcost[y0]);
+                        }
+                    }
                     total += nkt;
                     if (total > 2000000000LL)
                         return fail(ctx, MJK_EUNSUPPORTED, "quartet class has more than 2^31 tiles");
@@ -330,6 +341,7 @@ int build_tiles(mjk_basis* b, double cutoff)
             qc.nblocks = total;
             qc.work = (double)total * ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) * ncart(qc.ld) *
                       ((qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1);
+            qc.est = flops_class(qc.la, qc.lb, qc.lc, qc.ld, est_q, est_p);
             if (qc.nbt > 0) {
                 int rc;
                 if ((rc = upload(ctx, bt, &qc.bt))) return rc;
@@ -736,6 +748,7 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
         qa.np = p1 - p0;
         qa.qout = dq;
         qa.rys = ctx->rys;
+        qa.omega = b->omega;
         if (launch_qcond(la, lb, qa, ctx->stream)) {
             dfree(ctx, dq);
             return fail(ctx, MJK_EDEVICE, "q_cond launch failed for class (%d,%d)", la, lb);
@@ -872,6 +885,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     if (const char* rb = getenv("MJK_ROW_BLOCKS")) ctx->row_blocks = std::max(256, std::min(1 << 20, atoi(rb)));
     if (const char* rl = getenv("MJK_ROW_LDS")) ctx->row_lds = (size_t)std::max(0, std::min(61440, atoi(rl)));
     if (const char* ns = getenv("MJK_STREAMS")) ctx->nstreams = std::max(1, std::min(1 + mjk_ctx::NAUX, atoi(ns)));
+    if (const char* sc = getenv("MJK_SCHED")) ctx->sched = atoi(sc);
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     bool streams_ok = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; i < mjk_ctx::NAUX; ++i)
@@ -945,6 +959,17 @@ int mjk_basis_upload_batch(mjk_ctx* ctx, int nmol, const int* natm, const int* n
     int rc = MJK_OK;
     for (int m = 0; m < nmol && rc == MJK_OK; ++m)
         rc = parse_molecule(ctx, atm[m], natm[m], bas[m], nbas[m], env[m], b->mols[m]);
+    // the operator of every two-electron integral, as libcint reads it
+    if (rc == MJK_OK) {
+        b->omega = env[0][PTR_RANGE_OMEGA];
+        for (int m = 1; m < nmol; ++m)
+            if (env[m][PTR_RANGE_OMEGA] != b->omega) {
+                rc = fail(ctx, MJK_EINVAL, "molecules of one batch must share env[PTR_RANGE_OMEGA]");
+                break;
+            }
+        if (rc == MJK_OK && !std::isfinite(b->omega))
+            rc = fail(ctx, MJK_EINVAL, "env[PTR_RANGE_OMEGA] is not finite");
+    }
     if (rc == MJK_OK) rc = build_basis(ctx, b, cutoff);
     if (rc != MJK_OK) {
         std::string keep = ctx->err;
@@ -1068,6 +1093,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.screen = screen ? 1 : 0;
     a.fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
     a.rys = ctx->rys;
+    a.omega = b->omega;
     a.pmol = b->d_pmol;
     a.nbucket = NBUCKET;
     // classes, heaviest first, each on the least loaded of the streams (they all
@@ -1080,14 +1106,15 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     }
     std::vector<size_t> corder(b->qcls.size());
     std::iota(corder.begin(), corder.end(), 0);
+    auto cost = [&](const QClass& q) { return ctx->sched == 0 ? q.work : q.est; };
     std::stable_sort(corder.begin(), corder.end(),
-                     [&](size_t u, size_t v) { return b->qcls[u].work > b->qcls[v].work; });
+                     [&](size_t u, size_t v) { return cost(b->qcls[u]) > cost(b->qcls[v]); });
     double load[1 + mjk_ctx::NAUX] = {0, 0, 0, 0};
     for (size_t c : corder) {
         const QClass& qc = b->qcls[c];
         if (qc.nblocks == 0) continue;
         const int si = (int)(std::min_element(load, load + nst) - load);
-        load[si] += qc.work;
+        load[si] += cost(qc);
         hipStream_t st = streams[si];
         if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c], st));
         a.bt = qc.bt;

@@ -113,6 +113,29 @@ struct RysCursor {
     }
 };
 
+// Range-separated operators, selected by omega (env[PTR_RANGE_OMEGA] of the
+// basis): 0 the Coulomb 1/r, > 0 erf(omega r)/r, < 0 erfc(|omega| r)/r.  The
+// erf kernel restricts the Rys variable to t^2 <= theta = omega^2 / (omega^2 +
+// rho), rho = pq / (p + q); substituting t = sqrt(theta) s gives the
+// full-range roots at theta T, scaled by theta, with weights scaled by
+// sqrt(theta).  erfc = 1 - erf takes two root sets: the full-range set (pass
+// 0) and the erf set with negated weights (pass 1).
+__host__ __device__ __forceinline__ int rsh_passes(double omega) { return omega < 0.0 ? 2 : 1; }
+
+// T of the root set of this pass, and the factors for its roots and weights.
+__host__ __device__ __forceinline__ void rsh_scale(double omega, double rho, int pass, double& T, double& sx,
+                                                   double& sw)
+{
+    sx = 1.0;
+    sw = 1.0;
+    if (omega == 0.0 || (omega < 0.0 && pass == 0)) return;
+    const double w2 = omega * omega;
+    const double th = w2 / (w2 + rho);
+    T *= th;
+    sx = th;
+    sw = omega > 0.0 ? sqrt(th) : -sqrt(th);
+}
+
 // Host-side helper: the table as one flat array (coefficients then Hermite).
 inline void rys_fill_buffer(double* buf)
 {

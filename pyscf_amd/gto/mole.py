@@ -13,8 +13,11 @@ Reference behaviour restated here:
   * Angstrom -> Bohr with BOHR = 0.52917721092   pyscf/data/nist.py:24
   * ``energy_nuc``                               pyscf/gto/mole.py:1526-1551
   * ``ao_loc_nr`` = cumsum((2l+1)*nctr)          pyscf/gto/moleintor.py:804-820
+  * ``omega`` / ``set_range_coulomb`` /
+    ``with_range_coulomb`` (env[PTR_RANGE_OMEGA])  pyscf/gto/mole.py:2930-2979
 """
 
+import contextlib
 import math
 import numpy
 
@@ -276,6 +279,43 @@ class Mole:
         ia = self._bas[i, ATOM_OF]
         p = self._atm[ia, PTR_COORD]
         return self._env[p:p + 3].copy()
+
+    # -- range-separated Coulomb (pyscf/gto/mole.py:2930-2979) -------------
+    def set_range_coulomb(self, omega):
+        """Operator of every two-electron integral: omega = 0 the Coulomb
+        1/r, omega > 0 the long-range erf(omega r)/r, omega < 0 the
+        short-range erfc(|omega| r)/r."""
+        if omega is not None:
+            self._env[PTR_RANGE_OMEGA] = omega
+
+    @property
+    def omega(self):
+        return float(self._env[PTR_RANGE_OMEGA])
+
+    @omega.setter
+    def omega(self, v):
+        self.set_range_coulomb(v)
+
+    @contextlib.contextmanager
+    def _omega_context(self, omega):
+        omega0 = float(self._env[PTR_RANGE_OMEGA])
+        self.set_range_coulomb(omega)
+        try:
+            yield self
+        finally:
+            self.set_range_coulomb(omega0)
+
+    def with_range_coulomb(self, omega):
+        """Temporary omega (None: leave the operator as it is)."""
+        if omega is None:
+            return contextlib.nullcontext(self)
+        return self._omega_context(omega)
+
+    def with_long_range_coulomb(self, omega):
+        return contextlib.nullcontext(self) if omega is None else self.with_range_coulomb(abs(omega))
+
+    def with_short_range_coulomb(self, omega):
+        return contextlib.nullcontext(self) if omega is None else self.with_range_coulomb(-abs(omega))
 
 
 def M(**kwargs):

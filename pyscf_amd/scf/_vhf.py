@@ -23,6 +23,8 @@ import numpy
 
 from pyscf_amd import lib as _lib
 
+PTR_RANGE_OMEGA = 8       # env slot of the two-electron operator (pyscf/gto/mole.py:88)
+
 _engines = {}
 _engine_lock = threading.Lock()
 
@@ -173,6 +175,7 @@ class VHFOpt:
         self.prescreen = prescreen
         self.direct_scf_tol = direct_scf_tol
         self.engine = engine or get_engine()
+        self.omega = float(mol._env[PTR_RANGE_OMEGA])   # the bounds are of this operator
         self.basis = self.engine.upload(mol._atm, mol._bas, mol._env, direct_scf_tol)
 
     @property
@@ -195,6 +198,9 @@ def direct(dms, atm, bas, env, vhfopt=None, hermi=0, cart=False, with_j=True, wi
         basis = get_engine().cached_basis(atm, bas, env)
         tol = 0.0
     else:
+        if float(env[PTR_RANGE_OMEGA]) != vhfopt.omega:
+            raise ValueError('vhfopt was built for omega=%g but env[PTR_RANGE_OMEGA]=%g'
+                             % (vhfopt.omega, float(env[PTR_RANGE_OMEGA])))
         basis = vhfopt.basis
         tol = vhfopt.direct_scf_tol
     if basis.naos[0] != nao:

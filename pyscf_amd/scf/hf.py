@@ -18,10 +18,12 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None, 
     hermi: 0 general dm, 1 symmetric (Hermitian) dm, 2 anti-symmetric.
     vhfopt: a ``_vhf.VHFOpt`` enables Schwarz x density screening at its
     ``direct_scf_tol``; ``None`` evaluates every quartet.
+    omega: None keeps the molecule's operator; 0 the Coulomb 1/r, > 0 the
+    long-range erf(omega r)/r, < 0 the short-range erfc(|omega| r)/r
+    (``mol.with_range_coulomb``, pyscf/scf/hf.py:1006).  A ``vhfopt`` must
+    have been built for the same omega.
     fp32_tol: > 0 evaluates small quartets in FP32 (mixed precision).
     """
-    if omega is not None and omega != 0:
-        raise NotImplementedError('range-separated Coulomb (omega != 0) is not built')
     dm = numpy.asarray(dm, order='C')
     dm_shape = dm.shape
     dm_dtype = dm.dtype
@@ -29,8 +31,9 @@ def get_jk(mol, dm, hermi=1, vhfopt=None, with_j=True, with_k=True, omega=None, 
     if dm_dtype == numpy.complex128:
         dm = numpy.vstack((dm.real, dm.imag)).reshape(-1, nao, nao)
         hermi = 0
-    vj, vk = _vhf.direct(dm, mol._atm, mol._bas, mol._env, vhfopt, hermi, getattr(mol, 'cart', False),
-                         with_j, with_k, fp32_tol=fp32_tol)
+    with mol.with_range_coulomb(omega):
+        vj, vk = _vhf.direct(dm, mol._atm, mol._bas, mol._env, vhfopt, hermi, getattr(mol, 'cart', False),
+                             with_j, with_k, fp32_tol=fp32_tol)
     if dm_dtype == numpy.complex128:
         if with_j:
             vj = vj.reshape((2,) + dm_shape)

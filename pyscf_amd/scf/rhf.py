@@ -101,11 +101,16 @@ class RHF:
         return _vhf.VHFOpt(mol or self.mol, direct_scf_tol=self.direct_scf_tol)
 
     def get_jk(self, mol=None, dm=None, hermi=1, with_j=True, with_k=True, omega=None):
+        """One screening context per omega, built under that operator
+        (pyscf/scf/hf.py:2057-2080)."""
         mol = mol or self.mol
         if self._opt is None:
-            self._opt = self.init_direct_scf(mol)
+            self._opt = {}
+        if self._opt.get(omega) is None:
+            with mol.with_range_coulomb(omega):
+                self._opt[omega] = self.init_direct_scf(mol)
         t0 = time.perf_counter()
-        vj, vk = hf.get_jk(mol, dm, hermi, self._opt, with_j, with_k, omega, fp32_tol=self.fp32_tol)
+        vj, vk = hf.get_jk(mol, dm, hermi, self._opt[omega], with_j, with_k, omega, fp32_tol=self.fp32_tol)
         self.jk_times.append(time.perf_counter() - t0)
         return vj, vk
 

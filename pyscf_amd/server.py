@@ -121,7 +121,7 @@ class JKServer:
                 del opt
             else:
                 mf = rhf.RHF(req.mol, direct_scf_tol=self.direct_scf_tol, fp32_tol=self.fp32_tol)
-                mf._opt = _vhf.VHFOpt(req.mol, direct_scf_tol=self.direct_scf_tol, engine=eng)
+                mf._opt = {None: _vhf.VHFOpt(req.mol, direct_scf_tol=self.direct_scf_tol, engine=eng)}
                 mf.conv_tol = self.conv_tol
                 e = mf.kernel()
                 out = (e, mf.converged)
