@@ -679,16 +679,24 @@ void orc_dm_cond(double *dm_cond, const double *dm, int nset, const int *ao_loc,
 }
 
 /* Quartet predicate of CVHFnrs8_prescreen (lib/vhf/optimizer.c:90-117). */
+/* Prescreen predicates: 0 CVHFnrs8_prescreen (lib/vhf/optimizer.c:90-117),
+ * 1 CVHFnrs8_vj_prescreen (:119-140), 2 CVHFnrs8_vk_prescreen (:142-167). */
+static int orc_prescreen_mode = 0;
+
+void orc_set_prescreen(int mode)
+{
+        orc_prescreen_mode = mode;
+}
+
 static int prescreen(int i, int j, int k, int l, const double *q, const double *dc, int n, double cut)
 {
         double qijkl = q[i * n + j] * q[k * n + l];
-        return qijkl > cut
-            && ((4 * dc[j * n + i] * qijkl > cut)
-             || (4 * dc[l * n + k] * qijkl > cut)
-             || (dc[j * n + k] * qijkl > cut)
-             || (dc[j * n + l] * qijkl > cut)
-             || (dc[i * n + k] * qijkl > cut)
-             || (dc[i * n + l] * qijkl > cut));
+        int jt = (4 * dc[j * n + i] * qijkl > cut) || (4 * dc[l * n + k] * qijkl > cut);
+        int kt = (dc[j * n + k] * qijkl > cut) || (dc[j * n + l] * qijkl > cut)
+              || (dc[i * n + k] * qijkl > cut) || (dc[i * n + l] * qijkl > cut);
+        if (orc_prescreen_mode == 1) return qijkl > cut && jt;
+        if (orc_prescreen_mode == 2) return qijkl > cut && kt;
+        return qijkl > cut && (jt || kt);
 }
 
 /* Direct J/K over unique s8 shell quartets.  Outputs the FULL J (symmetric)

@@ -147,13 +147,18 @@ def int1e(mol, which):
     return out
 
 
-class VHFOpt:
-    """Screening context: Schwarz q_cond once per molecule, dm_cond per call."""
+_PRESCREEN = {'CVHFnrs8_prescreen': 0, 'CVHFnrs8_vj_prescreen': 1, 'CVHFnrs8_vk_prescreen': 2}
 
-    def __init__(self, mol, direct_scf_tol=1e-13):
+
+class VHFOpt:
+    """Screening context: Schwarz q_cond once per molecule, dm_cond per call;
+    ``prescreen`` names the quartet predicate (lib/vhf/optimizer.c:90-167)."""
+
+    def __init__(self, mol, direct_scf_tol=1e-13, prescreen='CVHFnrs8_prescreen'):
         atm, bas, env, ao_loc = _mol_arrays(mol)
         self.mol = mol
         self.direct_scf_tol = direct_scf_tol
+        self.prescreen = prescreen
         nbas = len(bas)
         self.q_cond = numpy.zeros((nbas, nbas))
         lib().orc_q_cond(_p(self.q_cond), _p(ao_loc), _p(atm), ctypes.c_int(len(atm)),
@@ -180,6 +185,7 @@ def direct_full(mol, dms, vhfopt=None, with_j=True, with_k=True, pair_stride=1, 
         q = numpy.ascontiguousarray(vhfopt.q_cond)
         dc = vhfopt.dm_cond(dms)
         qp, dcp, cut = _p(q), _p(dc), vhfopt.direct_scf_tol
+        lib().orc_set_prescreen(ctypes.c_int(_PRESCREEN[vhfopt.prescreen]))
     else:
         qp = dcp = None
         cut = 0.
@@ -189,6 +195,7 @@ def direct_full(mol, dms, vhfopt=None, with_j=True, with_k=True, pair_stride=1, 
                              _p(ao_loc), _p(atm), ctypes.c_int(len(atm)),
                              _p(bas), ctypes.c_int(len(bas)), _p(env),
                              ctypes.c_int(pair_stride), ctypes.c_int(pair_offset))
+    lib().orc_set_prescreen(ctypes.c_int(0))
     return vj, vk, nq
 
 

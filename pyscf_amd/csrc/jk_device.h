@@ -155,9 +155,14 @@ __device__ __forceinline__ double quartet_dmax(const JKArgs& a, int mol, int p, 
     const double* __restrict__ dc = a.dm_cond + a.M.cbase[mol];
     const int ns = a.M.nsh[mol];
     const int i = a.P.shA[p], j = a.P.shB[p], k = a.P.shA[q], l = a.P.shB[q];
-    double d = 4.0 * fmax(dc[j * ns + i], dc[l * ns + k]);
-    d = fmax(d, fmax(dc[j * ns + k], dc[j * ns + l]));
-    d = fmax(d, fmax(dc[i * ns + k], dc[i * ns + l]));
+    // J terms only for J builds, K terms only for K builds (the J-only and
+    // K-only predicates, pyscf/lib/vhf/optimizer.c:119-167)
+    double d = 0.0;
+    if (a.n_j) d = 4.0 * fmax(dc[j * ns + i], dc[l * ns + k]);
+    if (a.n_k) {
+        d = fmax(d, fmax(dc[j * ns + k], dc[j * ns + l]));
+        d = fmax(d, fmax(dc[i * ns + k], dc[i * ns + l]));
+    }
     return d;
 }
 

@@ -148,7 +148,7 @@ struct mjk_ctx {
     // run concurrently; they fork from and join back into `stream`
     static constexpr int NAUX = 3;
     int nstreams = 4;                     // 1 + aux streams used, MJK_STREAMS
-    int sched = 1;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
+    int sched = 0;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[NAUX] = {nullptr, nullptr, nullptr};

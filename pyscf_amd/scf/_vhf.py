@@ -164,8 +164,11 @@ class VHFOpt:
     """Screening context (counterpart of _VHFOpt, pyscf/scf/_vhf.py:151-264).
 
     The Schwarz bound q_cond and the shell-pair data are built once on the
-    device; ``direct_scf_tol`` is the prescreen cutoff of
-    CVHFnrs8_prescreen (pyscf/lib/vhf/optimizer.c:90-117)."""
+    device; ``direct_scf_tol`` is the prescreen cutoff.  The predicate
+    follows the requested matrices: with J and K it is CVHFnrs8_prescreen
+    (pyscf/lib/vhf/optimizer.c:90-117), with J only CVHFnrs8_vj_prescreen,
+    with K only CVHFnrs8_vk_prescreen (:119-167), as SCF.get_jk swaps them
+    (pyscf/scf/hf.py:2069-2077)."""
 
     def __init__(self, mol, intor=None, prescreen='CVHFnrs8_prescreen', qcondname='CVHFnr_int2e_q_cond',
                  dmcondname='CVHFnr_dm_cond', direct_scf_tol=1e-14, engine=None):

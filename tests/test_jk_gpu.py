@@ -195,3 +195,33 @@ def test_quartet_count_matches_oracle(oracle):
     nq = opt.basis.last_stats['quartets']
     _, _, nq0 = oracle.direct_full(mol, dm[None], vhfopt=oracle.VHFOpt(mol, 1e-13), pair_stride=0)
     assert abs(nq - nq0) <= max(2, 1e-4 * nq0), (nq, nq0)
+
+
+@pytest.mark.parametrize('which', ['j', 'k'])
+def test_j_only_k_only_prescreens(oracle, which):
+    """J-only and K-only builds screen with CVHFnrs8_vj_prescreen /
+    CVHFnrs8_vk_prescreen: the engine evaluates the quartets those predicates
+    keep, and the matrix agrees with the unscreened one."""
+    print('prescreen', which, flush=True)
+    mol = gto.M(atom='''O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587
+                        O 3.1 0.2 0; H 3.1 -0.557 0.587; H 3.1 0.957 0.587
+                        O 0 2.9 0.3; H 0 2.143 0.887; H 0 3.657 0.887''', basis='def2-svp')
+    nao = mol.nao_nr()
+    c = numpy.random.default_rng(23).standard_normal((nao, 15))
+    dm = c @ c.T / 15
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-11)
+    wj, wk = which == 'j', which == 'k'
+    vj, vk = opt.basis.get_jk(dm[None], 1, wj, wk, 1e-11, stats=True)
+    nq = opt.basis.last_stats['quartets']
+    opt.basis.get_jk(dm[None], 1, True, True, 1e-11, stats=True)
+    nq_both = opt.basis.last_stats['quartets']
+    name = 'CVHFnrs8_vj_prescreen' if wj else 'CVHFnrs8_vk_prescreen'
+    _, _, nq0 = oracle.direct_full(mol, dm[None], vhfopt=oracle.VHFOpt(mol, 1e-11, prescreen=name),
+                                   pair_stride=0)
+    assert abs(nq - nq0) <= max(2, 1e-4 * nq0), (nq, nq0)
+    assert nq < nq_both
+    j0, k0 = oracle.get_jk(mol, dm, hermi=1)
+    if wj:
+        assert vk is None and abs(vj[0] - j0).max() < 1e-8
+    else:
+        assert vj is None and abs(vk[0].reshape(nao, nao) - k0).max() < 1e-8
