@@ -162,9 +162,7 @@ def bench_batched(args):
     # error from two SCF runs that settled in different solutions
     from pyscf_amd.scf import rhf
     outliers = []
-    for i in numpy.argsort(-de)[:5]:
-        if not (de[i] > 1e-8 and both[i]):
-            break
+    for i in [j for j in numpy.argsort(-de) if both[j] and de[j] > 1e-8][:5]:
         progress('c4: rechecking molecule %d alone (|dE| %.2e)' % (i, de[i]))
         es = []
         for t32 in (0.0, args.fp32_tol):

@@ -338,7 +338,7 @@ __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
                 for (int pass = 0; pass < rsh_passes(a.omega); ++pass) {
                     double Tp = T, sx, sw;
                     rsh_scale(a.omega, rho, pass, Tp, sx, sw);
-                    const RysCursor<NR> rys(a.rys, Tp);
+                    const typename RysFor<NR, LA + LB + LC + LD>::type rys(a.rys, Tp);
 #pragma unroll 1
                     for (int r = 0; r < NR; ++r) {
                         double rx, rw;
@@ -356,7 +356,7 @@ __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
                 const int t0 = c.tq + ik * c.ts, t1 = t0 + c.ts;
                 const PrimQuartet m0 = prim_quartet(Pp, Kp, a.P.tP[t0], a.P.tK[t0], c, T0);
                 const PrimQuartet m1 = prim_quartet(Pp, Kp, a.P.tP[t1], a.P.tK[t1], c, T1);
-                const RysCursor<NR> rys0(a.rys, T0), rys1(a.rys, T1);
+                const typename RysFor<NR, LA + LB + LC + LD>::type rys0(a.rys, T0), rys1(a.rys, T1);
 #pragma unroll 1
                 for (int r = 0; r < NR; ++r) {
                     double rx0, rw0, rx1, rw1;
@@ -371,7 +371,7 @@ __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
             double T;
             const int t0 = c.tq + ik * c.ts;
             const PrimQuartet m = prim_quartet(Pp, Kp, a.P.tP[t0], a.P.tK[t0], c, T);
-            const RysCursor<NR> rys(a.rys, T);
+            const typename RysFor<NR, LA + LB + LC + LD>::type rys(a.rys, T);
 #pragma unroll 1
             for (int r = 0; r < NR; ++r) {
                 double rx, rw;

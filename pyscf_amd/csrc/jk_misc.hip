@@ -38,7 +38,7 @@ __global__ __launch_bounds__(64) void k_qcond(const QCondArgs a)
             for (int pass = 0; pass < rsh_passes(a.omega); ++pass) {   // range-separated operators
             double Tp = T, sx, sw;
             rsh_scale(a.omega, Pp.w * Qp.w * ipq, pass, Tp, sx, sw);
-            const RysCursor<NR> rys(a.rys, Tp);
+            const typename RysFor<NR, 2 * (LA + LB)>::type rys(a.rys, Tp);
 #pragma unroll 1
             for (int r = 0; r < NR; ++r) {
                 double rx, rw;
