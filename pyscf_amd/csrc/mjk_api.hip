@@ -4,7 +4,7 @@
 //   * every contracted shell with nctr > 1 is split into nctr segmented
 //     shells sharing the primitives, keeping PySCF's AO order (ctr, m);
 //   * shell pairs (i >= j) are built with the higher l first, primitive pairs
-//     with exp(-ab/(a+b)|AB|^2) below exp(-EXPCUTOFF) are dropped, and the
+//     with exp(-ab/(a+b)|AB|^2) below exp(-expcutoff) are dropped, and the
 //     prefactor sqrt(2) pi^(5/4) ca cb exp(-ab/p AB^2)/p is folded per pair;
 //   * Schwarz bounds are computed on the GPU (k_qcond), pairs are sorted by
 //     class and descending bound, and per quartet class a list of bra tiles
@@ -56,7 +56,13 @@ using namespace mjk;
 
 namespace {
 
-constexpr double EXPCUTOFF = 60.0;   // drop primitive pairs with exp(-ab/p AB^2) < e^-60
+// Primitive pairs with exp(-ab/p AB^2) below e^-40 = 4e-18 are dropped (libcint
+// drops them below e^-60).  Measured on (H2O)16 def2-SVP: 7% fewer primitive
+// quartets, 111 -> 115 builds/s, J and K unchanged beyond 5e-14 (the
+// summation-order noise of two builds); e^-30 moved J by 8e-10.  The largest
+// cutoff an override may set (MJK_EXPCUTOFF) is libcint's 60.
+constexpr double EXPCUTOFF_DEFAULT = 40.0;
+constexpr double EXPCUTOFF_MAX = 60.0;
 constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8, PTR_COORD = 1, ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2,
               NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6, PTR_RANGE_OMEGA = 8;
 constexpr int NQCLASS_MAX = NPAIRCLASS * (NPAIRCLASS + 1) / 2;
@@ -150,6 +156,7 @@ struct mjk_ctx {
     static constexpr int NAUX = 3;
     int nstreams = 4;                     // 1 + aux streams used, MJK_STREAMS
     int sched = 0;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
+    double expcutoff = EXPCUTOFF_DEFAULT; // primitive pairs with exp(-ab/p AB^2) < e^-expcutoff dropped; MJK_EXPCUTOFF
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
     hipEvent_t ev_join[NAUX] = {nullptr, nullptr, nullptr};
@@ -621,7 +628,7 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
                         const double ea = sa.e[ia], eb = sbb.e[ib];
                         const double p = ea + eb;
                         const double E = ea * eb / p * rr;
-                        if (E > EXPCUTOFF) continue;
+                        if (E > ctx->expcutoff) continue;
                         const double cc = sa.c[ia] * sbb.c[ib];
                         if (cc == 0.0) continue;
                         double4 P;
@@ -889,6 +896,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     if (const char* rl = getenv("MJK_ROW_LDS")) ctx->row_lds = (size_t)std::max(0, std::min(61440, atoi(rl)));
     if (const char* ns = getenv("MJK_STREAMS")) ctx->nstreams = std::max(1, std::min(1 + mjk_ctx::NAUX, atoi(ns)));
     if (const char* sc = getenv("MJK_SCHED")) ctx->sched = atoi(sc);
+    if (const char* ec = getenv("MJK_EXPCUTOFF")) ctx->expcutoff = std::max(20.0, std::min(EXPCUTOFF_MAX, atof(ec)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     bool streams_ok = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; i < mjk_ctx::NAUX; ++i)
