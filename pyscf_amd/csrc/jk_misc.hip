@@ -236,8 +236,24 @@ __global__ __launch_bounds__(256) void k_copy(const double* __restrict__ src, do
         dst[i] = src[i];
 }
 
+// Zero fill (a kernel rather than a memset node, so that a captured graph
+// replays it like any other kernel).
+__global__ __launch_bounds__(256) void k_zero(double* __restrict__ dst, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dst[i] = 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
+int launch_zero(double* dst, long long n, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    const long long nb = std::min<long long>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)nb), dim3(256), 0, st, dst, n);
+    return (int)hipGetLastError();
+}
+
 int launch_copy(const double* src, double* dst, long long n, hipStream_t st)
 {
     if (n <= 0) return 0;

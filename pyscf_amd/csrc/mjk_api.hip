@@ -50,6 +50,7 @@ int launch_dmcond(const DmCondArgs& x, const double* Dsph, long long sslot, int 
                   hipStream_t st);
 int lmax_kernels();
 int launch_copy(const double* src, double* dst, long long n, hipStream_t st);
+int launch_zero(double* dst, long long n, hipStream_t st);
 }  // namespace mjk
 
 using namespace mjk;
@@ -156,6 +157,7 @@ struct mjk_ctx {
     static constexpr int NAUX = 3;
     int nstreams = 4;                     // 1 + aux streams used, MJK_STREAMS
     int sched = 0;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
+    int graphs = 1;                       // replay plain builds as HIP graphs; MJK_GRAPHS=0 launches eagerly
     double expcutoff = EXPCUTOFF_DEFAULT; // primitive pairs with exp(-ab/p AB^2) < e^-expcutoff dropped; MJK_EXPCUTOFF
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
@@ -217,10 +219,32 @@ int upload(mjk_ctx* ctx, const std::vector<T>& v, T** d)
 
 }  // namespace
 
+// Everything a captured build depends on: a replay is valid only for the same key.
+struct GraphKey {
+    const void* dms = nullptr;
+    const void* vj = nullptr;
+    const void* vk = nullptr;
+    const void* bufs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int n_dm = 0, hermi = 0, with_j = 0, with_k = 0, nstreams = 0;
+    double cut = 0, fp32_tol = 0;
+    size_t row_lds = 0;
+    bool operator==(const GraphKey& o) const
+    {
+        for (int i = 0; i < 5; ++i)
+            if (bufs[i] != o.bufs[i]) return false;
+        return dms == o.dms && vj == o.vj && vk == o.vk && n_dm == o.n_dm && hermi == o.hermi &&
+               with_j == o.with_j && with_k == o.with_k && nstreams == o.nstreams && cut == o.cut &&
+               fp32_tol == o.fp32_tol && row_lds == o.row_lds;
+    }
+};
+
 struct mjk_basis {
     mjk_ctx* ctx = nullptr;
     int nmol = 0;
     double omega = 0;   // two-electron operator, env[PTR_RANGE_OMEGA]: 0 1/r, > 0 erf, < 0 erfc
+    // the last plain build (no statistics, no profiling) as an instantiated HIP graph
+    hipGraphExec_t gexec = nullptr;
+    GraphKey gkey;
     std::vector<MolHost> mols;
     int maxld = 0, maxnao = 0, maxnsh = 0;
     long long slot_s = 0, slot_c = 0, cond_total = 0;
@@ -268,8 +292,15 @@ struct mjk_basis {
 
 namespace {
 
+void drop_graph(mjk_basis* b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+}
+
 void free_tiles(mjk_basis* b)
 {
+    drop_graph(b);   // its kernels read the task lists
     for (auto& c : b->qcls) {
         dfree(b->ctx, c.bt);
         dfree(b->ctx, c.bt_mol);
@@ -397,11 +428,13 @@ int build_tiles(mjk_basis* b, double cutoff)
                                                   std::max(cost_all / ctx->row_min_blocks, 1LL));
                 std::vector<std::pair<long long, int4>> tasks;
                 std::vector<int> tprim;   // bra primitive offset of every task (parallel to tasks)
-                // bra primitive pairs per task: small classes share their bra pairs out
-                // finely (a small molecule's long contractions are its critical path);
-                // large ones in eights (measured on MI355X, (H2O)16 def2-SVP with the
-                // classes on four streams: 2 -> 92, 4 -> 107, 8 -> 110, 16 -> 110 builds/s)
-                const int P = ctx->row_prim > 0 ? ctx->row_prim : cost_all < (1LL << 20) ? 2 : 8;
+                // bra primitive pairs per task: small classes give every bra primitive
+                // pair its own task (a small molecule's long contractions are its
+                // critical path: H2O cc-pVDZ 0.69 -> 0.52 ms per build from 2 to 1);
+                // large ones share them in eights (measured on MI355X, (H2O)16 def2-SVP
+                // with the classes on four streams: 2 -> 92, 4 -> 107, 8 -> 110 builds/s)
+                const int P = ctx->row_prim > 0 ? ctx->row_prim
+                              : cost_all < (1LL << 18) ? 1 : cost_all < (1LL << 20) ? 2 : 8;
                 qc.rprim = P;
                 for (size_t ib = 0; ib < bra.size(); ++ib)
                 for (int pbo = 0; pbo < b->h_nprim[bra[ib]]; pbo += P) {
@@ -896,6 +929,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     if (const char* rl = getenv("MJK_ROW_LDS")) ctx->row_lds = (size_t)std::max(0, std::min(61440, atoi(rl)));
     if (const char* ns = getenv("MJK_STREAMS")) ctx->nstreams = std::max(1, std::min(1 + mjk_ctx::NAUX, atoi(ns)));
     if (const char* sc = getenv("MJK_SCHED")) ctx->sched = atoi(sc);
+    if (const char* gr = getenv("MJK_GRAPHS")) ctx->graphs = atoi(gr) != 0;
     if (const char* ec = getenv("MJK_EXPCUTOFF")) ctx->expcutoff = std::max(20.0, std::min(EXPCUTOFF_MAX, atof(ec)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     bool streams_ok = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
@@ -1071,18 +1105,24 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     const bool need_cond = screen || fp32_tol > 0;
     if (need_cond && (rc = ensure(ctx, &b->dmcond, b->cap_cond, b->cond_total))) return rc;
     hipStream_t st = ctx->stream;
-    HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
-    if (n_j) HIPCHK(ctx, hipMemsetAsync(b->J, 0, sizeof(double) * ncopy * jcopy, st));
-    if (n_k) HIPCHK(ctx, hipMemsetAsync(b->K, 0, sizeof(double) * ncopy * kcopy, st));
     const bool prof = b->profiling && b->pev.size() >= 2 * b->qcls.size();
     const bool tally = stats || (prof && b->prof_counts);
-    if (tally) HIPCHK(ctx, hipMemsetAsync(b->counters, 0, sizeof(unsigned long long) * 4 * NQCLASS_MAX, st));
+    // The whole build on the context's stream and the streams it forks.
+    // capture: recorded into a graph, without the timing events.  The
+    // accumulators are cleared by a kernel, not a memset node.
+    auto enqueue = [&](bool capture) -> int {
+    if (!capture) HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
+    if (n_j && launch_zero(b->J, ncopy * jcopy, st)) return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
+    if (n_k && launch_zero(b->K, ncopy * kcopy, st)) return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
+    // (the 64-bit counters are cleared as doubles: both are all-zero bits)
+    if (tally && launch_zero(reinterpret_cast<double*>(b->counters), 4 * NQCLASS_MAX, st))
+        return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
     if (need_cond && launch_dmcond(dmcond_args(b), d_dms, b->slot_s, n_dm, b->dmcond, st))
         return fail(ctx, MJK_EDEVICE, "dm_cond launch failed");
     if (launch_sph2cart(xform(b), b->maxld, d_dms, b->slot_s, n_dm, hermi, with_j, with_k, b->Dj, b->Dk,
                         b->slot_c, st))
         return fail(ctx, MJK_EDEVICE, "density transform launch failed");
-    HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
+    if (!capture) HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
     JKArgs a;
     a.P = pairset(b);
     a.M.dbase = b->d_dbase;
@@ -1155,11 +1195,54 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         HIPCHK(ctx, hipEventRecord(ctx->ev_join[i - 1], streams[i]));
         HIPCHK(ctx, hipStreamWaitEvent(st, ctx->ev_join[i - 1], 0));
     }
-    HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
+    if (!capture) HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
     if (launch_cart2sph(xform(b), b->maxnao, b->J, b->K, b->slot_c, n_dm, hermi, with_j, with_k, d_vj, d_vk,
                         b->slot_s, ncopy, jcopy, kcopy, st))
         return fail(ctx, MJK_EDEVICE, "output transform launch failed");
-    HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
+    if (!capture) HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
+    return MJK_OK;
+    };
+    // Plain builds (no statistics, no profiling) replay one HIP graph of the
+    // whole build while nothing it depends on changes: one launch instead of
+    // one per kernel (the launch-bound small molecules gain most).
+    if (ctx->graphs && !prof && !tally) {
+        GraphKey key;
+        key.dms = d_dms;
+        key.vj = with_j ? d_vj : nullptr;
+        key.vk = with_k ? d_vk : nullptr;
+        const void* bufs[5] = {b->J, b->K, b->Dj, b->Dk, b->dmcond};
+        for (int i = 0; i < 5; ++i) key.bufs[i] = bufs[i];
+        key.n_dm = n_dm;
+        key.hermi = hermi;
+        key.with_j = with_j;
+        key.with_k = with_k;
+        key.nstreams = ctx->nstreams;
+        key.cut = cut;
+        key.fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
+        key.row_lds = ctx->row_lds;
+        if (!(b->gexec && b->gkey == key)) {
+            drop_graph(b);
+            HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rc = enqueue(true);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc == MJK_OK && e != hipSuccess)
+                rc = fail(ctx, MJK_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+            if (rc == MJK_OK) {
+                const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+                if (ei != hipSuccess) {
+                    b->gexec = nullptr;
+                    rc = fail(ctx, MJK_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+                }
+            }
+            if (g) (void)hipGraphDestroy(g);
+            if (rc != MJK_OK) return rc;
+            b->gkey = key;
+        }
+        HIPCHK(ctx, hipGraphLaunch(b->gexec, st));
+        return MJK_OK;
+    }
+    if ((rc = enqueue(false))) return rc;
     if (prof) {
         HIPCHK(ctx, hipStreamSynchronize(st));
         float span = 0;
