@@ -156,8 +156,9 @@ struct mjk_ctx {
     // run concurrently; they fork from and join back into `stream`
     static constexpr int NAUX = 3;
     int nstreams = 4;                     // 1 + aux streams used, MJK_STREAMS
-    int sched = 0;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
+    int sched = 1;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
     int graphs = 1;                       // replay plain builds as HIP graphs; MJK_GRAPHS=0 launches eagerly
+    int order = 1;                        // issue order within a stream: 0 by cost, 1 heavy classes first; MJK_ORDER
     double expcutoff = EXPCUTOFF_DEFAULT; // primitive pairs with exp(-ab/p AB^2) < e^-expcutoff dropped; MJK_EXPCUTOFF
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
@@ -930,6 +931,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     if (const char* ns = getenv("MJK_STREAMS")) ctx->nstreams = std::max(1, std::min(1 + mjk_ctx::NAUX, atoi(ns)));
     if (const char* sc = getenv("MJK_SCHED")) ctx->sched = atoi(sc);
     if (const char* gr = getenv("MJK_GRAPHS")) ctx->graphs = atoi(gr) != 0;
+    if (const char* od = getenv("MJK_ORDER")) ctx->order = atoi(od);
     if (const char* ec = getenv("MJK_EXPCUTOFF")) ctx->expcutoff = std::max(20.0, std::min(EXPCUTOFF_MAX, atof(ec)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     bool streams_ok = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
@@ -1160,13 +1162,31 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     auto cost = [&](const QClass& q) { return ctx->sched == 0 ? q.work : q.est; };
     std::stable_sort(corder.begin(), corder.end(),
                      [&](size_t u, size_t v) { return cost(b->qcls[u]) > cost(b->qcls[v]); });
-    double load[1 + mjk_ctx::NAUX] = {0, 0, 0, 0};
-    for (size_t c : corder) {
+    std::vector<int> cstream(b->qcls.size(), -1);
+    {
+        double load[1 + mjk_ctx::NAUX] = {0, 0, 0, 0};
+        for (size_t c : corder) {
+            if (b->qcls[c].nblocks == 0) continue;
+            const int si = (int)(std::min_element(load, load + nst) - load);
+            load[si] += cost(b->qcls[c]);
+            cstream[c] = si;
+        }
+    }
+    // issue order (the order within each stream): the register-heavy classes (most
+    // Cartesian components x roots) first, so that they hold CUs before the light
+    // classes fill them (MJK_ORDER=0: by cost)
+    std::vector<size_t> lorder = corder;
+    if (ctx->order == 1) {
+        auto heavy = [&](const QClass& q) {
+            return ncart(q.la) * ncart(q.lb) * ncart(q.lc) * ncart(q.ld) * ((q.la + q.lb + q.lc + q.ld) / 2 + 1);
+        };
+        std::stable_sort(lorder.begin(), lorder.end(),
+                         [&](size_t u, size_t v) { return heavy(b->qcls[u]) > heavy(b->qcls[v]); });
+    }
+    for (size_t c : lorder) {
         const QClass& qc = b->qcls[c];
-        if (qc.nblocks == 0) continue;
-        const int si = (int)(std::min_element(load, load + nst) - load);
-        load[si] += cost(qc);
-        hipStream_t st = streams[si];
+        if (cstream[c] < 0) continue;
+        hipStream_t st = streams[cstream[c]];
         if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c], st));
         a.bt = qc.bt;
         a.bt_mol = qc.bt_mol;
