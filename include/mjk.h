@@ -81,6 +81,15 @@ size_t mjk_ctx_hbm_used(const mjk_ctx* ctx);
  * 61440; at most 61440.  Results agree either way to rounding. */
 int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes);
 
+/* No reference counterpart (a launch-overhead option): when enabled, a build
+ * without statistics or profiling is captured once into a HIP graph and
+ * replayed while its inputs, flags and buffers are unchanged.  Off by
+ * default: a capture fails (the build returns MJK_EDEVICE) if another thread
+ * of the process issues work that synchronises with the context's streams
+ * meanwhile, e.g. on the legacy null stream; enable it only where the
+ * context's thread owns the device. */
+int mjk_ctx_set_graphs(mjk_ctx* ctx, int enable);
+
 /* Upload one molecule's basis and build its shell pairs, Schwarz bounds
  * (the q_cond of CVHFnr_int2e_q_cond, pyscf/lib/vhf/optimizer.c:408-454)
  * and quartet tile lists for screening at `cutoff` (<= 0: none).

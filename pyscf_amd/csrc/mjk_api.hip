@@ -157,7 +157,7 @@ struct mjk_ctx {
     static constexpr int NAUX = 3;
     int nstreams = 4;                     // 1 + aux streams used, MJK_STREAMS
     int sched = 1;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
-    int graphs = 1;                       // replay plain builds as HIP graphs; MJK_GRAPHS=0 launches eagerly
+    int graphs = 0;                       // replay plain builds as HIP graphs (mjk_ctx_set_graphs, MJK_GRAPHS)
     int order = 1;                        // issue order within a stream: 0 by cost, 1 heavy classes first; MJK_ORDER
     double expcutoff = EXPCUTOFF_DEFAULT; // primitive pairs with exp(-ab/p AB^2) < e^-expcutoff dropped; MJK_EXPCUTOFF
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
@@ -989,6 +989,14 @@ int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes)
     if (bytes > 61440) return fail(ctx, MJK_EINVAL, "row kernel LDS limit is 61440 bytes");
     std::lock_guard<std::recursive_mutex> lock(ctx->mu);
     ctx->row_lds = bytes;
+    return MJK_OK;
+}
+
+int mjk_ctx_set_graphs(mjk_ctx* ctx, int enable)
+{
+    if (!ctx) return MJK_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    ctx->graphs = enable != 0;
     return MJK_OK;
 }
 

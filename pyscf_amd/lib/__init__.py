@@ -40,6 +40,7 @@ def _declare(lib):
     lib.mjk_ctx_hbm_used.argtypes = [c_void_p]
     lib.mjk_ctx_hbm_used.restype = c_size_t
     lib.mjk_ctx_set_row_lds.argtypes = [c_void_p, c_size_t]
+    lib.mjk_ctx_set_graphs.argtypes = [c_void_p, c_int]
     lib.mjk_basis_upload.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_double, P(c_void_p)]
     lib.mjk_basis_upload_batch.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_double, P(c_void_p)]

@@ -37,6 +37,7 @@ def test_bad_arguments_return_codes():
     assert l.mjk_ctx_create(None, 0, 0) == -1
     assert l.mjk_last_error(None) == b'no context'
     assert l.mjk_ctx_set_row_lds(None, 0) == -1
+    assert l.mjk_ctx_set_graphs(None, 1) == -1
 
 
 @pytest.mark.parametrize('n', range(1, 10))

@@ -240,7 +240,16 @@ def test_graph_replays_match_eager(oracle):
     c1, c2 = rng.standard_normal((nao, 10)), rng.standard_normal((nao, 10))
     dms = [c1 @ c1.T / 10, c2 @ c2.T / 10]
     opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    eng = opt.engine
     ref = [opt.basis.get_jk(d[None], 1, True, True, 1e-13, stats=True) for d in dms]
+    assert eng.lib.mjk_ctx_set_graphs(eng.handle, 1) == 0
+    try:
+        _graph_replays(oracle, mol, opt, dms, ref)
+    finally:
+        assert eng.lib.mjk_ctx_set_graphs(eng.handle, 0) == 0
+
+
+def _graph_replays(oracle, mol, opt, dms, ref):
     for rep in range(4):
         for i in (0, 1):
             j, k = opt.basis.get_jk(dms[i][None], 1, True, True, 1e-13)          # graph
