@@ -85,9 +85,9 @@ int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes);
  * without statistics or profiling is captured once into a HIP graph and
  * replayed while its inputs, flags and buffers are unchanged.  Off by
  * default: a capture fails (the build returns MJK_EDEVICE) if another thread
- * of the process issues work that synchronises with the context's streams
- * meanwhile, e.g. on the legacy null stream; enable it only where the
- * context's thread owns the device. */
+ * of the process synchronises the device meanwhile (hipFree does, and so
+ * does destroying another basis handle); enable it only where no other
+ * thread frees device memory during builds. */
 int mjk_ctx_set_graphs(mjk_ctx* ctx, int enable);
 
 /* Upload one molecule's basis and build its shell pairs, Schwarz bounds
