@@ -101,3 +101,38 @@ def test_row_and_fallback_kernels_agree(oracle, omega):
     with mol.with_range_coulomb(omega):
         j0, k0 = oracle.get_jk(mol, dm, hermi=0, vhfopt=oracle.VHFOpt(mol, 1e-13))
     assert abs(j_row - j0).max() < TOL * 10 and abs(k_row - k0).max() < TOL * 10
+
+
+def test_mixed_precision_and_batches(oracle):
+    """The range-separated operators with FP32 recurrences for small quartets,
+    in a batch of two molecules; a batch whose molecules disagree on omega is
+    refused."""
+    print('rsh mixed + batch', flush=True)
+    m1 = gto.M(atom=H2O, basis='def2-svp')
+    m2 = gto.M(atom='O 0 0 0; H 0 0 0.96; H 0.93 0 -0.25', basis='631g')
+    eng = _vhf.get_engine()
+    for m in (m1, m2):
+        m.omega = 0.3
+    b = eng.upload_batch([m1, m2], 1e-13)
+    dms = [_dm(m, 5, 3 + i) for i, m in enumerate((m1, m2))]
+    flat = numpy.zeros((1, b.slot))      # the molecules' nao^2 blocks back to back
+    flat[0, :dms[0].size] = dms[0].ravel()
+    flat[0, dms[0].size:dms[0].size + dms[1].size] = dms[1].ravel()
+    vj, vk = b.get_jk(flat, 1, True, True, 1e-13, fp32_tol=1e-7)
+    off = 0
+    for m, d in zip((m1, m2), dms):
+        n = d.size
+        j0, k0 = oracle.get_jk(m, d, hermi=1)
+        assert abs(vj[0, off:off + n].reshape(d.shape) - j0).max() < 1e-8
+        assert abs(vk[0, off:off + n].reshape(d.shape) - k0).max() < 1e-8
+        off += n
+    m2.omega = -0.3
+    with pytest.raises(RuntimeError):
+        eng.upload_batch([m1, m2], 1e-13)
+    for m in (m1, m2):
+        m.omega = 0.0
+
+
+def _dm(mol, nocc, seed):
+    c = numpy.random.default_rng(seed).standard_normal((mol.nao_nr(), nocc))
+    return c @ c.T
