@@ -249,7 +249,7 @@ def bench_serving(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print('bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md); '
+        print('bench.py runs on one GPU only: the J/K engine has no multi-GPU path; '
               'got --gpus %d' % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.workload == 'c4':
