@@ -4,7 +4,7 @@ FP64 tolerance: 1e-10 max-abs on J and K (the target for the FP64 path)."""
 import numpy
 import pytest
 
-from pyscf_amd import gto, lib
+from pyscf_amd import gto, lib, workloads
 from pyscf_amd.scf import _vhf, hf
 from tests.conftest import H2O
 
@@ -172,7 +172,7 @@ def test_row_and_tile_kernels_agree(oracle, row_lds):
     try:
         got = [hf.get_jk(mol, dms[0], hermi=1, vhfopt=opt), hf.get_jk(mol, dms, hermi=0, vhfopt=opt)]
     finally:
-        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 61440) == 0
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 98304) == 0
     for (j0, k0), (j1, k1) in zip(ref, got):
         assert abs(j1 - j0).max() < 1e-11 * max(1, abs(j0).max())
         assert abs(k1 - k0).max() < 1e-11 * max(1, abs(k0).max())
@@ -264,3 +264,28 @@ def _graph_replays(oracle, mol, opt, dms, ref):
     assert abs(k - ref[0][1]).max() < 1e-12 * abs(ref[0][1]).max()
     j0, k0 = oracle.get_jk(mol, dms[0], hermi=1)
     assert abs(j - j0).max() < 1e-10 and abs(k - k0).max() < 1e-10
+
+
+def test_row_kernels_above_64k_lds(oracle):
+    """Four non-symmetric densities give the (dd|..) row kernels K' rows of
+    8 x 12 x ld doubles, more than 64 KB: they run with the 160 KB limit and
+    agree with the fallback kernels and the oracle."""
+    print('row kernels > 64 KB LDS', flush=True)
+    mol = gto.M(atom=workloads.water_cluster(4, seed=3), basis='def2-svp')
+    nao = mol.nao_nr()
+    rng = numpy.random.default_rng(31)
+    dms = rng.standard_normal((4, nao, nao))
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    eng = _vhf.get_engine()
+    out = {}
+    try:
+        for lds in (161792, 0):
+            assert eng.lib.mjk_ctx_set_row_lds(eng.handle, lds) == 0
+            out[lds] = hf.get_jk(mol, dms, hermi=0, vhfopt=opt)
+    finally:
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 98304) == 0
+    (j1, k1), (j0, k0) = out[161792], out[0]
+    assert abs(j1 - j0).max() < 1e-11 * max(1, abs(j0).max())
+    assert abs(k1 - k0).max() < 1e-11 * max(1, abs(k0).max())
+    jo, ko = oracle.get_jk(mol, dms[1], hermi=0)
+    assert abs(j1[1] - jo).max() < 1e-9 and abs(k1[1] - ko).max() < 1e-9

@@ -95,7 +95,7 @@ def test_row_and_fallback_kernels_agree(oracle, omega):
     try:
         j_fb, k_fb = hf.get_jk(mol, dm, hermi=0, vhfopt=opt, omega=omega)
     finally:
-        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 61440) == 0
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 98304) == 0
     assert abs(j_row - j_fb).max() < 1e-11 * max(1, abs(j_fb).max())
     assert abs(k_row - k_fb).max() < 1e-11 * max(1, abs(k_fb).max())
     with mol.with_range_coulomb(omega):
