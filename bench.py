@@ -1,23 +1,40 @@
 """Benchmark of the direct J/K build (get_jk) on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload c2] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload c3] [--no-cpu] [--no-oracle]
 
-Workload (default "c2"): (H2O)16 on a jittered 3.0 Angstrom cubic lattice,
-def2-SVP (384 AOs), RHF converged to 1e-10 Eh with the GPU engine; then one
-step = one J/K build (J and K of the converged density, hermi=1,
-direct_scf_tol=1e-13) with the density already resident in HBM.  Every
-step is synchronised so that its latency is a sample of the p50/p99.
+Default workload "c3": all-trans C30H62, cc-pVDZ (730 AOs).  RHF is
+converged to 1e-10 Eh with the GPU engine; then one step = one J/K build of
+the converged density (J and K, hermi=1, direct_scf_tol=1e-13) with the
+density already resident in HBM.  Every step is synchronised, so that its
+latency is a sample of the p50/p99.
 
-Printed: one JSON line with value = JK builds/s (whole job, one GPU),
-quartets/s, p50/p99 latency, the roofline of the dominant quartet-class
-kernel (algorithmic FP64 FLOPs / its HIP-event duration on the launch
-stream, against 78.6 TFLOP/s) and the CPU baseline (the oracle's OpenMP
-restatement of the reference's direct J/K loop, timed on a bounded sample of
-bra shell pairs of the same build and scaled to builds/s).
+The JSON line (stdout) carries:
+  * value = FP64 JK builds/s (one GPU), quartets/s, p50/p99;
+  * mixed: the same builds with FP32 recurrences for quartets bounded below
+    fp32_tol = 1e-7 (builds/s, speed-up) and the mixed-precision RHF energy
+    against the FP64 one;
+  * accuracy: max |dJ|, max |dK| of the FP64 and the mixed builds against
+    one oracle build of the same density (the oracle's OpenMP restatement of
+    the reference's direct J/K loop, on all host cores);
+  * roofline: the quartet-class kernel with the most algorithmic FP64 FLOPs,
+    its FLOPs per launch over its average duration from HIP events on its
+    launch stream during the timed builds, against 78.6 TFLOP/s (the kernels
+    run on the FP64 vector ALU); also its average over every launch of the
+    process (what a kernel-trace average of the same run reports);
+  * cpu_baseline: the oracle timed on a bounded sample of bra shell pairs of
+    the same screened build (1 warm-up, median of 5), scaled by quartet count;
+  * secondary: the (H2O)16 def2-SVP build (384 AOs) with its oracle check.
+
+Other modes: --workload c1/c2 (single molecules), c4 (256 chains batched,
+active-molecule builds/s against the same molecules' own RHF runs), c5
+(serving stream); --pmc-run DM.npy (two builds of a saved density and a
+64 MiB calibration copy, for rocprofv3 --pmc runs); --pmc-summary FETCH_DIR
+WRITE_DIR (per-kernel HBM bytes per launch from those runs' CSVs).
 """
 
 import argparse
 import ctypes
+import glob
 import json
 import os
 import sys
@@ -30,14 +47,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'JK builds/s + ERI quartets/s, mixed vs FP64, at <=1e-8 Eh; p50/p99 latency'
 FP64_PEAK = 78.6e12
+TOL = 1e-13
+_T0 = time.perf_counter()
 
 
 def progress(msg):
     """One progress line on stderr (the JSON line alone goes to stdout)."""
     print('[bench %.0fs] %s' % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
-
-
-_T0 = time.perf_counter()
 
 
 def kernel_name(cls, kind, rsh=False):
@@ -55,11 +71,10 @@ def kernel_name(cls, kind, rsh=False):
     return 'mjk::k_jk<%d, %d, %d, %d, %d, 0>' % (la, lb, lc, ld, jk_split(la, lb, lc, ld))
 
 
-def traffic_bytes(kernel):
-    """HBM bytes per launch of `kernel` on the c2 workload, from the calibrated
-    FETCH_SIZE and WRITE_SIZE counter runs summarised in profiles/ (None if
-    the kernel has no entry there)."""
-    path = os.path.join(ROOT, 'profiles', 'jk_c2_traffic.json')
+def traffic_bytes(kernel, workload):
+    """HBM bytes per launch of `kernel` from the calibrated FETCH_SIZE and
+    WRITE_SIZE summary of this workload in profiles/ (None without one)."""
+    path = os.path.join(ROOT, 'profiles', 'jk_%s_traffic.json' % workload)
     try:
         with open(path) as f:
             rec = json.load(f)['kernels'][kernel]
@@ -73,21 +88,27 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='c2', choices=['c1', 'c2', 'c3', 'c4', 'c5'])
+    ap.add_argument('--workload', default='c3', choices=['c1', 'c2', 'c3', 'c4', 'c5'])
     ap.add_argument('--nmol', type=int, default=256, help='c4: molecules in the batch')
     ap.add_argument('--nreq', type=int, default=48, help='c5: requests in the stream')
     ap.add_argument('--rate', type=float, default=20.0, help='c5: Poisson arrival rate (1/s)')
     ap.add_argument('--max-ao', type=int, default=1500, help='c5: largest request (AOs)')
-    ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--dm-out', default=None, help='c2: save the converged density (.npy)')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--no-oracle', action='store_true', help='skip the oracle accuracy check')
+    ap.add_argument('--no-secondary', action='store_true', help='skip the (H2O)16 secondary line')
+    ap.add_argument('--dm-out', default=None, help='save the converged density (.npy)')
     ap.add_argument('--dm-in', default=None,
-                    help='c2: load the density from a .npy written by --dm-out instead of running the SCF '
+                    help='load the density from a .npy written by --dm-out instead of running the SCF '
                          '(profiling runs: every J/K launch is then one of the converged density)')
-    ap.add_argument('--cpu-seconds', type=float, default=10.0)
+    ap.add_argument('--cpu-seconds', type=float, default=3.0, help='CPU baseline: seconds per timed sample')
     ap.add_argument('--fp32-tol', type=float, default=1e-7)
     ap.add_argument('--omega', type=float, default=0.0,
                     help='c1-c3: time J/K builds of the range-separated operator (> 0 erf, < 0 erfc) '
                          'on the converged Coulomb density')
+    ap.add_argument('--pmc-run', default=None, metavar='DM.npy',
+                    help='counter runs: two builds of the saved density, then a 64 MiB calibration copy')
+    ap.add_argument('--pmc-summary', nargs=2, default=None, metavar=('FETCH_DIR', 'WRITE_DIR'),
+                    help='HBM bytes per launch and kernel from the FETCH_SIZE and WRITE_SIZE runs')
     return ap.parse_args()
 
 
@@ -101,34 +122,296 @@ def workload(name):
     return gto.M(atom=workloads.alkane(30), basis='cc-pvdz'), 'C30H62 all-trans, cc-pVDZ (730 AOs)'
 
 
-def cpu_baseline(mol, dm, tol, seconds):
-    """Oracle (OpenMP C) on a bounded sample of bra shell pairs."""
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def oracle_module():
+    """The oracle, on every host core with threads bound close (set before its
+    OpenMP runtime starts), with interpolated roots."""
+    os.environ.setdefault('OMP_PROC_BIND', 'close')
     from oracle import oracle as O
+    O.set_threads(len(os.sched_getaffinity(0)))
+    O.set_roots(1)
+    return O
+
+
+def oracle_jk(mol, dm):
+    O = oracle_module()
+    t = time.perf_counter()
+    opt = O.VHFOpt(mol, TOL)
+    j0, k0 = O.get_jk(mol, dm, hermi=1, vhfopt=opt)
+    return j0, k0, time.perf_counter() - t
+
+
+def cpu_baseline(mol, dm, seconds):
+    """The oracle on a bounded sample of bra shell pairs of the same screened
+    build: stride chosen for about `seconds` per run, 1 warm-up, median of 5,
+    scaled to whole builds by the quartet count."""
+    O = oracle_module()
     ncores = len(os.sched_getaffinity(0))
-    O.set_threads(ncores)
-    opt = O.VHFOpt(mol, tol)
+    opt = O.VHFOpt(mol, TOL)
     dms = dm[None]
     _, _, n_total = O.direct_full(mol, dms, opt, pair_stride=0)
     t = time.perf_counter()
     _, _, n_probe = O.direct_full(mol, dms, opt, pair_stride=97)
     rate = n_probe / max(time.perf_counter() - t, 1e-6)
     stride = max(1, int(n_total / max(rate * seconds, 1)))
-    t = time.perf_counter()
-    _, _, n_s = O.direct_full(mol, dms, opt, pair_stride=stride)
-    dt = time.perf_counter() - t
+    O.direct_full(mol, dms, opt, pair_stride=stride)           # warm-up
+    runs = []
+    for _ in range(5):
+        t = time.perf_counter()
+        _, _, n_s = O.direct_full(mol, dms, opt, pair_stride=stride)
+        runs.append(time.perf_counter() - t)
+    dt = float(numpy.median(runs))
     qps = n_s / dt
     return {'value': qps / n_total, 'unit': 'JK builds/s', 'cores': ncores, 'kind': 'port',
-            'quartets_per_s': qps,
-            'sample': 'bra shell pairs ij with ij %% %d == 0 of the same screened build: %d of %d '
-                      'unique shell quartets in %.2f s, scaled by quartet count' % (stride, n_s, n_total, dt)}
+            'cpu_model': cpu_model(), 'quartets_per_s': qps, 'runs_s': runs,
+            'sample': 'bra shell pairs ij with ij %% %d == 0 of the same screened build (%d of %d unique shell '
+                      'quartets), oracle with interpolated Rys roots, OMP_PROC_BIND=close, 1 warm-up, median of '
+                      '5 runs %.2f s, scaled by quartet count' % (stride, n_s, n_total, dt)}
+
+
+class DeviceBuild:
+    """The converged density resident in HBM and timed mjk_get_jk_device builds."""
+
+    def __init__(self, basis, dm):
+        from pyscf_amd.scf import _vhf  # noqa: F401
+        self.basis = basis
+        self.eng = basis.engine
+        self.L = self.eng.lib
+        self.ctx = self.eng.handle
+        self.dm = dm
+        self.nbytes = dm.nbytes
+        self.bufs = [ctypes.c_void_p() for _ in range(3)]
+        for p in self.bufs:
+            self.eng.check(self.L.mjk_device_alloc(self.ctx, self.nbytes, ctypes.byref(p)), 'device_alloc')
+        self.eng.check(self.L.mjk_memcpy_h2d(self.ctx, self.bufs[0], self.ptr(dm), self.nbytes), 'h2d')
+
+    @staticmethod
+    def ptr(a):
+        return a.ctypes.data_as(ctypes.c_void_p)
+
+    def free(self):
+        for p in self.bufs:
+            self.L.mjk_device_free(self.ctx, p)
+
+    def build(self, fp32_tol, stats=None):
+        d_dm, d_vj, d_vk = self.bufs
+        rc = self.L.mjk_get_jk_device(self.ctx, self.basis.handle, d_dm, 1, 1, 1, 1, TOL, fp32_tol, d_vj, d_vk,
+                                      ctypes.byref(stats) if stats is not None else None)
+        self.eng.check(rc, 'mjk_get_jk_device')
+
+    def sync(self):
+        self.eng.check(self.L.mjk_synchronize(self.ctx), 'sync')
+
+    def read_profile(self):
+        L, ctx, h, p = self.L, self.ctx, self.basis.handle, self.ptr
+        maxc = 64
+        cls = numpy.zeros(4 * maxc, dtype=numpy.int32)
+        ms, fl = numpy.zeros(maxc), numpy.zeros(maxc)
+        nq, npq, nl = (numpy.zeros(maxc, dtype=numpy.int64) for _ in range(3))
+        k = L.mjk_profile_read(ctx, h, maxc, p(cls), p(ms), p(nq), p(npq), p(fl), p(nl))
+        kinds = numpy.zeros(maxc, dtype=numpy.int32)
+        L.mjk_profile_kinds(ctx, h, maxc, p(kinds))
+        span, nspan = ctypes.c_double(), ctypes.c_longlong()
+        self.eng.check(L.mjk_profile_span(ctx, h, ctypes.byref(span), ctypes.byref(nspan)), 'profile_span')
+        out = {}
+        for i in range(k):
+            key = tuple(int(x) for x in cls[4 * i:4 * i + 4])
+            out[key] = {'ms': ms[i], 'launches': int(nl[i]), 'nq': int(nq[i]), 'flops': fl[i], 'kind': int(kinds[i])}
+        return out, span.value, nspan.value
+
+    def timed(self, fp32_tol, steps, warmup):
+        """One counted build (per-class work), warm-up builds, then `steps`
+        synchronised builds; HIP events time every class launch of all of
+        them (profiling mode 2 keeps no tallies)."""
+        from pyscf_amd import lib
+        L, ctx, h = self.L, self.ctx, self.basis.handle
+        st = lib.mjk_stats()
+        self.eng.check(L.mjk_profile(ctx, h, 1), 'profile')
+        self.build(fp32_tol, st)                  # tile lists for this cutoff, work buffers, tallies
+        counted, _, _ = self.read_profile()
+        self.eng.check(L.mjk_profile(ctx, h, 2), 'profile')
+        for _ in range(warmup):
+            self.build(fp32_tol)
+        self.sync()
+        warm, _, _ = self.read_profile()
+        self.eng.check(L.mjk_profile(ctx, h, 2), 'profile')
+        lat = []
+        self.sync()
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            t = time.perf_counter()
+            self.build(fp32_tol)
+            self.sync()
+            lat.append(time.perf_counter() - t)
+        t_total = time.perf_counter() - t_start
+        prof, span, nspan = self.read_profile()
+        L.mjk_profile(ctx, h, 0)
+        classes = []
+        for key, r in prof.items():
+            c = counted.get(key, {'nq': 0, 'flops': 0.0, 'ms': 0.0, 'launches': 0})
+            w = warm.get(key, {'ms': 0.0, 'launches': 0})
+            t1 = r['ms'] / r['launches']
+            n_all = r['launches'] + c['launches'] + w['launches']
+            classes.append({'class': '(%d%d|%d%d)' % key, 'kernel': kernel_name(key, r['kind'], self.basis_omega()),
+                            'ms_per_launch': t1,
+                            'ms_per_launch_all_launches': (r['ms'] + c['ms'] + w['ms']) / max(n_all, 1),
+                            'launches_all': n_all, 'quartets': c['nq'], 'flops': c['flops'],
+                            'tflops': c['flops'] / (t1 * 1e-3) / 1e12 if t1 > 0 else 0.0})
+        classes.sort(key=lambda c: -c['ms_per_launch'])
+        vj, vk = numpy.empty_like(self.dm), numpy.empty_like(self.dm)
+        self.eng.check(L.mjk_memcpy_d2h(ctx, self.ptr(vj), self.bufs[1], self.nbytes), 'd2h')
+        self.eng.check(L.mjk_memcpy_d2h(ctx, self.ptr(vk), self.bufs[2], self.nbytes), 'd2h')
+        return {'t_total': t_total, 'lat': lat, 'classes': classes, 'stats': st, 'vj': vj, 'vk': vk,
+                'span_ms': span / max(nspan, 1), 'steps': steps}
+
+    def basis_omega(self):
+        return bool(getattr(self.basis, 'omega', 0.0))
+
+
+def single_molecule(args, name, steps, warmup, with_cpu, with_oracle, fp32_tol):
+    """SCF, timed FP64 (and mixed) builds, oracle accuracy, CPU baseline."""
+    from pyscf_amd.scf import _vhf, rhf
+    mol, desc = workload(name)
+    nao = mol.nao_nr()
+    t0 = time.perf_counter()
+    if args.dm_in:
+        dm = numpy.ascontiguousarray(numpy.load(args.dm_in, allow_pickle=False))
+        opt = _vhf.VHFOpt(mol, direct_scf_tol=TOL)
+        e64 = None
+        scf_info = {'skipped': 'density from %s' % os.path.basename(args.dm_in)}
+    else:
+        progress('%s: FP64 RHF' % name)
+        mf = rhf.RHF(mol, direct_scf_tol=TOL)
+        mf.conv_tol = 1e-10
+        e64 = mf.kernel()
+        dm = numpy.ascontiguousarray(mf.dm)
+        opt = mf._opt[None]
+        scf_info = {'e_tot': e64, 'cycles': mf.cycles, 'converged': mf.converged,
+                    'seconds': time.perf_counter() - t0}
+        if args.dm_out:
+            numpy.save(args.dm_out, dm)
+    if args.omega:
+        with mol.with_range_coulomb(args.omega):
+            opt = _vhf.VHFOpt(mol, direct_scf_tol=TOL)
+        opt.basis.omega = args.omega
+    dev = DeviceBuild(opt.basis, dm)
+    progress('%s: %d timed FP64 builds' % (name, steps))
+    r64 = dev.timed(0.0, steps, warmup)
+    rmx = None
+    if fp32_tol > 0:
+        progress('%s: %d timed mixed builds' % (name, steps))
+        rmx = dev.timed(fp32_tol, steps, warmup)
+    dev.free()
+    emx = None
+    if rmx is not None and e64 is not None and not args.omega:
+        progress('%s: mixed-precision RHF' % name)
+        mfm = rhf.RHF(mol, direct_scf_tol=TOL, fp32_tol=fp32_tol)
+        mfm.conv_tol = 1e-10
+        emx = mfm.kernel()
+    accuracy = None
+    if with_oracle:
+        progress('%s: oracle build of the same density' % name)
+        with mol.with_range_coulomb(args.omega or None):
+            j0, k0, t_orc = oracle_jk(mol, dm)
+        accuracy = {'reference': 'oracle (CPU restatement of the reference direct J/K), same density and '
+                                 'screening', 'oracle_build_s': t_orc,
+                    'fp64_max_abs_dJ': float(abs(r64['vj'] - j0).max()),
+                    'fp64_max_abs_dK': float(abs(r64['vk'] - k0).max())}
+        if rmx is not None:
+            accuracy['mixed_max_abs_dJ'] = float(abs(rmx['vj'] - j0).max())
+            accuracy['mixed_max_abs_dK'] = float(abs(rmx['vk'] - k0).max())
+    cpu = None
+    if with_cpu:
+        progress('%s: CPU baseline' % name)
+        with mol.with_range_coulomb(args.omega or None):
+            cpu = cpu_baseline(mol, dm, args.cpu_seconds)
+    return {'mol': mol, 'desc': desc, 'nao': nao, 'scf': scf_info, 'e64': e64, 'emx': emx,
+            'r64': r64, 'rmx': rmx, 'accuracy': accuracy, 'cpu': cpu}
+
+
+def main_single(args):
+    res = single_molecule(args, args.workload, args.steps, args.warmup, not args.no_cpu, not args.no_oracle,
+                          args.fp32_tol)
+    r64, rmx = res['r64'], res['rmx']
+    t_total, lat, classes, st = r64['t_total'], r64['lat'], r64['classes'], r64['stats']
+    # the classes run concurrently on several streams: the dominant kernel is the
+    # one with the most algorithmic FLOPs; its duration is measured while it
+    # shares the GPU with the others
+    dom = max(classes, key=lambda c: c['flops'])
+    span_ms = r64['span_ms']
+    builds = args.steps / t_total
+    total_flops = sum(c['flops'] for c in classes)
+    out = {
+        'metric': METRIC, 'value': builds, 'unit': 'JK builds/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': t_total / args.steps * 1e3, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic geometry; converged RHF density from this engine',
+        'config': {'workload': res['desc'] + ', one J/K build of the converged RHF density, hermi=1, '
+                   'direct_scf_tol=1e-13', 'nao': res['nao'], 'n_dm': 1, 'precision': 'fp64',
+                   'omega': args.omega},
+        'quartets_per_s': st.quartets * builds, 'quartets_per_build': int(st.quartets),
+        'p50_ms': float(numpy.percentile(lat, 50) * 1e3), 'p99_ms': float(numpy.percentile(lat, 99) * 1e3),
+        'scf': res['scf'],
+        'gpu_quartet_phase_ms_per_build': span_ms,
+        'fp64_tflops_all_classes': total_flops / (span_ms * 1e-3) / 1e12,
+        'roofline': {'bound': 'valu', 'compute': 'FP64 vector ALU (no MFMA: Rys recurrences)',
+                     'kernel': dom['kernel'], 'class': dom['class'],
+                     'achieved': dom['tflops'], 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
+                     'frac': dom['tflops'] * 1e12 / FP64_PEAK,
+                     'traffic': traffic_bytes(dom['kernel'], args.workload),
+                     'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch'],
+                     'ms_per_launch_all_launches': dom['ms_per_launch_all_launches'],
+                     'launches_all': dom['launches_all'],
+                     'build': {'what': 'all quartet-class kernels of one J/K build, concurrent on 4 streams',
+                               'achieved': total_flops / (span_ms * 1e-3) / 1e12,
+                               'frac': total_flops / (span_ms * 1e-3) / FP64_PEAK,
+                               'flops': total_flops, 'ms': span_ms}},
+        'classes': classes[:10],
+    }
+    if rmx is not None:
+        b_mx = args.steps / rmx['t_total']
+        out['mixed'] = {
+            'fp32_tol': args.fp32_tol, 'jk_builds_per_s': b_mx, 'speedup_vs_fp64': b_mx / builds,
+            'quartets_per_s': rmx['stats'].quartets * b_mx,
+            'fp32_quartet_fraction': rmx['stats'].quartets_fp32 / max(rmx['stats'].quartets, 1),
+            'p50_ms': float(numpy.percentile(rmx['lat'], 50) * 1e3),
+            'p99_ms': float(numpy.percentile(rmx['lat'], 99) * 1e3),
+            'max_abs_dJ_vs_fp64_gpu': float(abs(rmx['vj'] - r64['vj']).max()),
+            'max_abs_dK_vs_fp64_gpu': float(abs(rmx['vk'] - r64['vk']).max()),
+            'e_tot': res['emx'], 'dE_vs_fp64': res['emx'] - res['e64'] if res['emx'] is not None else None,
+        }
+    if res['accuracy'] is not None:
+        out['accuracy'] = res['accuracy']
+    if res['cpu'] is not None:
+        out['cpu_baseline'] = res['cpu']
+    if args.workload == 'c3' and not args.no_secondary and not args.dm_in and not args.omega:
+        progress('secondary: (H2O)16 def2-SVP')
+        sec = single_molecule(args, 'c2', args.steps, args.warmup, False, not args.no_oracle, 0.0)
+        s64 = sec['r64']
+        out['secondary'] = {'workload': sec['desc'], 'jk_builds_per_s': args.steps / s64['t_total'],
+                            'ms_per_build': s64['t_total'] / args.steps * 1e3,
+                            'p50_ms': float(numpy.percentile(s64['lat'], 50) * 1e3),
+                            'quartets_per_build': int(s64['stats'].quartets),
+                            'scf': sec['scf'], 'accuracy': sec['accuracy']}
+    print(json.dumps(out))
 
 
 def bench_batched(args):
-    """256 random-walk chains (def2-SVP): batched SCF (one launch per quartet
-    class and iteration for all molecules) vs the same J/K builds issued one
-    molecule at a time."""
+    """256 random-walk chains (def2-SVP): batched SCF (one graph-replayed build
+    per iteration for all molecules still active) against the same molecules'
+    own RHF runs, one molecule per build; both count active-molecule builds
+    (each a density-difference build of one molecule)."""
     from pyscf_amd import gto, workloads
-    from pyscf_amd.scf import _vhf, batch
+    from pyscf_amd.scf import batch, rhf
     mols = [gto.M(atom=workloads.random_chain([4, i]), basis='def2-svp') for i in range(args.nmol)]
     naos = [m.nao_nr() for m in mols]
     res = {}
@@ -140,54 +423,42 @@ def bench_batched(args):
         e = b.kernel()
         t_scf = time.perf_counter() - t0
         t_jk = sum(b.build_times)
+        n_act = sum(b.build_active)
+        cyc = [{'active': a, 'ms': 1e3 * t} for a, t in zip(b.build_active, b.build_times)]
         res[label] = {'e_tot': e, 'scf_s': t_scf, 'iterations': b.cycles, 'jk_s': t_jk,
                       'converged': int(b.converged.sum()), 'conv_mask': numpy.array(b.converged),
-                      'mol_builds_per_s': len(mols) * len(b.build_times) / t_jk}
-    # unbatched baseline: the FP64 J/K build of every molecule, one call each
-    dms = b.dms
-    eng = _vhf.get_engine()
-    progress('c4: unbatched baseline, %d uploads' % len(mols))
-    opts = [_vhf.VHFOpt(m, direct_scf_tol=1e-13, engine=eng) for m in mols]
-    for o, d in zip(opts, dms):
-        o.basis.get_jk(d[None], 1, True, True, 1e-13)
-    t0 = time.perf_counter()
-    for o, d in zip(opts, dms):
-        o.basis.get_jk(d[None], 1, True, True, 1e-13)
-    t_single = time.perf_counter() - t0
-    single = len(mols) / t_single
+                      'active_builds': n_act, 'active_builds_per_s': n_act / t_jk, 'per_build': cyc}
+    progress('c4: unbatched baseline, each molecule its own FP64 RHF')
+    n_single, t_single, e_single = 0, 0.0, []
+    for m in mols:
+        mf = rhf.RHF(m, direct_scf_tol=TOL)
+        mf.conv_tol = 1e-9
+        e_single.append(mf.kernel())
+        n_single += len(mf.jk_times)
+        t_single += sum(mf.jk_times)
+    single = n_single / t_single
+    e_single = numpy.array(e_single)
     de = numpy.abs(res['mixed']['e_tot'] - res['fp64']['e_tot'])
     both = res['mixed']['conv_mask'] & res['fp64']['conv_mask']
-    # molecules whose batched mixed and FP64 energies differ by more than 1e-8:
-    # rerun them alone (FP64 and mixed, tighter convergence) to tell a precision
-    # error from two SCF runs that settled in different solutions
-    from pyscf_amd.scf import rhf
-    outliers = []
-    for i in [j for j in numpy.argsort(-de) if both[j] and de[j] > 1e-8][:5]:
-        progress('c4: rechecking molecule %d alone (|dE| %.2e)' % (i, de[i]))
-        es = []
-        for t32 in (0.0, args.fp32_tol):
-            mf = rhf.RHF(mols[i], direct_scf_tol=1e-13, fp32_tol=t32)
-            mf.conv_tol = 1e-10
-            es.append(mf.kernel())
-        outliers.append({'molecule': int(i), 'batched_dE': float(de[i]), 'alone_fp64': es[0], 'alone_mixed': es[1],
-                         'alone_dE': es[1] - es[0], 'batched_fp64': float(res['fp64']['e_tot'][i]),
-                         'batched_mixed': float(res['mixed']['e_tot'][i])})
+    de_single = numpy.abs(res['fp64']['e_tot'] - e_single)
+    pb = res['mixed']['per_build']
     out = {
-        'metric': METRIC, 'value': res['mixed']['mol_builds_per_s'], 'unit': 'molecule JK builds/s',
+        'metric': METRIC, 'value': res['mixed']['active_builds_per_s'], 'unit': 'active-molecule JK builds/s',
         'n_gpus': 1, 'steps': res['mixed']['iterations'], 'warmup': 0,
         'ms_per_step': 1e3 * res['mixed']['jk_s'] / res['mixed']['iterations'], 'higher_is_better': True,
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64+f32', 'data': 'synthetic random-walk chains',
         'config': {'workload': '%d random-walk chains (3-12 heavy atoms, seeds [4,i]), def2-SVP, '
                    '%d-%d AOs, batched RHF to 1e-9' % (len(mols), min(naos), max(naos))},
-        'fp64_batched_mol_builds_per_s': res['fp64']['mol_builds_per_s'],
-        'mixed_batched_mol_builds_per_s': res['mixed']['mol_builds_per_s'],
-        'unbatched_fp64_mol_builds_per_s': single,
-        'speedup_mixed_batched_vs_unbatched_fp64': res['mixed']['mol_builds_per_s'] / single,
-        'speedup_fp64_batched_vs_unbatched_fp64': res['fp64']['mol_builds_per_s'] / single,
+        'fp64_batched_active_builds_per_s': res['fp64']['active_builds_per_s'],
+        'mixed_batched_active_builds_per_s': res['mixed']['active_builds_per_s'],
+        'unbatched_fp64_builds_per_s': single,
+        'unbatched_builds': n_single, 'unbatched_jk_s': t_single,
+        'speedup_mixed_batched_vs_unbatched_fp64': res['mixed']['active_builds_per_s'] / single,
+        'speedup_fp64_batched_vs_unbatched_fp64': res['fp64']['active_builds_per_s'] / single,
+        'first_build': pb[0], 'last_builds': pb[-5:],
         'max_abs_dE_mixed_vs_fp64': float(de[both].max()) if both.any() else None,
-        'max_abs_dE_mixed_vs_fp64_incl_unconverged': float(de.max()),
         'molecules_dE_over_1e-8': int((de[both] > 1e-8).sum()),
-        'dE_outliers_rerun_alone': outliers,
+        'max_abs_dE_batched_vs_single_fp64': float(de_single[res['fp64']['conv_mask']].max()),
         'converged': [res['fp64']['converged'], res['mixed']['converged']],
         'scf_seconds': [res['fp64']['scf_s'], res['mixed']['scf_s']],
     }
@@ -246,187 +517,84 @@ def bench_serving(args):
     print(json.dumps(out))
 
 
+def pmc_run(args):
+    """Two J/K builds of a saved density of the workload, then a 64 MiB device
+    copy (kernel k_copy) whose byte counts calibrate the HBM counters: it moves
+    exactly 64 MiB each way."""
+    from pyscf_amd.scf import _vhf
+    mol, _ = workload(args.workload)
+    dm = numpy.ascontiguousarray(numpy.load(args.pmc_run, allow_pickle=False))
+    assert dm.shape == (mol.nao_nr(), mol.nao_nr())
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=TOL)
+    eng = opt.engine
+    eng.check(eng.lib.mjk_ctx_set_graphs(eng.handle, 0), 'set_graphs')   # kernels appear by name
+    for _ in range(2):
+        opt.basis.get_jk(dm[None], 1, True, True, TOL)
+    lib, ctx = eng.lib, eng.handle
+    nb = 64 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    eng.check(lib.mjk_device_alloc(ctx, nb, ctypes.byref(a)), 'alloc')
+    eng.check(lib.mjk_device_alloc(ctx, nb, ctypes.byref(b)), 'alloc')
+    eng.check(lib.mjk_device_copy(ctx, b, a, nb), 'copy')
+    eng.check(lib.mjk_synchronize(ctx), 'sync')
+    print('pmc run done', flush=True)
+
+
+def _pmc_table(d, counter):
+    """{kernel: [sum of counter per dispatch]} from a rocprofv3 --pmc CSV dir."""
+    import csv
+    files = glob.glob(os.path.join(d, '**', '*counter_collection.csv'), recursive=True)
+    if not files:
+        raise SystemExit('no counter_collection.csv under %s' % d)
+    per = {}
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                if row.get('Counter_Name') != counter:
+                    continue
+                key = (row['Kernel_Name'].split('(')[0], row['Dispatch_Id'])
+                per[key] = per.get(key, 0.0) + float(row['Counter_Value'])
+    out = {}
+    for (k, _), v in per.items():
+        out.setdefault(k, []).append(v)
+    return out
+
+
+def pmc_summary(args):
+    fetch = _pmc_table(args.pmc_summary[0], 'FETCH_SIZE')
+    write = _pmc_table(args.pmc_summary[1], 'WRITE_SIZE')
+    copy = [k for k in fetch if 'k_copy' in k][0]
+    fs = (64 << 20) / (fetch[copy][0] * 1024)      # KiB per dispatch -> bytes, calibrated
+    ws = (64 << 20) / (write[copy][0] * 1024)
+    kernels = {}
+    for k in fetch:
+        if 'k_jk' not in k or k not in write:
+            continue
+        kernels[k] = {'launches': len(fetch[k]),
+                      'fetch_bytes_per_launch': fs * 1024 * float(numpy.mean(fetch[k])),
+                      'write_bytes_per_launch': ws * 1024 * float(numpy.mean(write[k]))}
+    print(json.dumps({'workload': args.workload, 'fetch_scale': fs, 'write_scale': ws,
+                      'calibration': 'k_copy moves 64 MiB each way; FETCH_SIZE read %.0f KiB (scale %.4f), '
+                                     'WRITE_SIZE read %.0f KiB (scale %.4f)'
+                                     % (fetch[copy][0], fs, write[copy][0], ws),
+                      'kernels': kernels}, indent=1))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: the J/K engine has no multi-GPU path; '
               'got --gpus %d' % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.pmc_summary:
+        return pmc_summary(args)
+    if args.pmc_run:
+        return pmc_run(args)
     if args.workload == 'c4':
         return bench_batched(args)
     if args.workload == 'c5':
         return bench_serving(args)
-    from pyscf_amd import lib
-    from pyscf_amd.scf import _vhf, rhf
-
-    mol, desc = workload(args.workload)
-    nao = mol.nao_nr()
-    tol = 1e-13
-    t0 = time.perf_counter()
-    if args.dm_in:
-        dm = numpy.ascontiguousarray(numpy.load(args.dm_in, allow_pickle=False))
-        opt = _vhf.VHFOpt(mol, direct_scf_tol=tol)
-        e64 = None
-        scf_info = {'skipped': 'density from %s' % os.path.basename(args.dm_in)}
-    else:
-        mf = rhf.RHF(mol, direct_scf_tol=tol)
-        mf.conv_tol = 1e-10
-        e64 = mf.kernel()
-        dm = numpy.ascontiguousarray(mf.dm)
-        opt = mf._opt[None]
-        scf_info = {'e_tot': e64, 'cycles': mf.cycles, 'converged': mf.converged,
-                    'seconds': time.perf_counter() - t0}
-        if args.dm_out:
-            numpy.save(args.dm_out, dm)
-
-    if args.omega:
-        # bounds and builds of the range-separated operator, on the same density
-        with mol.with_range_coulomb(args.omega):
-            opt = _vhf.VHFOpt(mol, direct_scf_tol=tol)
-    eng = opt.engine
-    L = eng.lib
-    basis = opt.basis
-    ctx = eng.handle
-    nbytes = dm.nbytes
-    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    d_dm, d_vj, d_vk = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
-    for p in (d_dm, d_vj, d_vk):
-        eng.check(L.mjk_device_alloc(ctx, nbytes, ctypes.byref(p)), 'device_alloc')
-    eng.check(L.mjk_memcpy_h2d(ctx, d_dm, ptr(dm), nbytes), 'h2d')
-
-    def timed(fp32_tol):
-        """warmup, then args.steps synchronised builds with per-class HIP events."""
-        st = lib.mjk_stats()
-        def build(stats=None):
-            rc = L.mjk_get_jk_device(ctx, basis.handle, d_dm, 1, 1, 1, 1, tol, fp32_tol, d_vj, d_vk,
-                                     ctypes.byref(stats) if stats is not None else None)
-            eng.check(rc, 'mjk_get_jk_device')
-        build(st)                      # first call: tile lists for this cutoff, work buffers
-        for _ in range(args.warmup):
-            build()
-        eng.check(L.mjk_synchronize(ctx), 'sync')
-        maxc = 64
-        cls = numpy.zeros(4 * maxc, dtype=numpy.int32)
-        ms = numpy.zeros(maxc)
-        nq = numpy.zeros(maxc, dtype=numpy.int64)
-        npq = numpy.zeros(maxc, dtype=numpy.int64)
-        fl = numpy.zeros(maxc)
-        nl = numpy.zeros(maxc, dtype=numpy.int64)
-        # work per class from one counted build (untimed) ...
-        eng.check(L.mjk_profile(ctx, basis.handle, 1), 'profile')
-        build()
-        k = L.mjk_profile_read(ctx, basis.handle, maxc, ptr(cls), ptr(ms), ptr(nq), ptr(npq), ptr(fl), ptr(nl))
-        work = {tuple(cls[4 * i:4 * i + 4]): (int(nq[i] // nl[i]), fl[i] / nl[i]) for i in range(k)}
-        # ... and kernel times from HIP events over the timed builds (no tallies kept)
-        eng.check(L.mjk_profile(ctx, basis.handle, 2), 'profile')
-        lat = []
-        eng.check(L.mjk_synchronize(ctx), 'sync')
-        t_start = time.perf_counter()
-        for _ in range(args.steps):
-            t = time.perf_counter()
-            build()
-            eng.check(L.mjk_synchronize(ctx), 'sync')
-            lat.append(time.perf_counter() - t)
-        eng.check(L.mjk_synchronize(ctx), 'sync')
-        t_total = time.perf_counter() - t_start
-        k = L.mjk_profile_read(ctx, basis.handle, maxc, ptr(cls), ptr(ms), ptr(nq), ptr(npq), ptr(fl), ptr(nl))
-        kinds = numpy.zeros(maxc, dtype=numpy.int32)
-        L.mjk_profile_kinds(ctx, basis.handle, maxc, ptr(kinds))
-        span, nspan = ctypes.c_double(), ctypes.c_longlong()
-        eng.check(L.mjk_profile_span(ctx, basis.handle, ctypes.byref(span), ctypes.byref(nspan)), 'profile_span')
-        L.mjk_profile(ctx, basis.handle, 0)
-        classes = []
-        for i in range(k):
-            key = tuple(int(x) for x in cls[4 * i:4 * i + 4])
-            nq1, fl1 = work.get(key, (0, 0.0))
-            t1 = ms[i] / nl[i]
-            kname = kernel_name(key, int(kinds[i]), bool(args.omega))
-            classes.append({'class': '(%d%d|%d%d)' % key, 'kernel': kname,
-                            'ms_per_launch': t1, 'quartets': nq1, 'flops': fl1,
-                            'tflops': fl1 / (t1 * 1e-3) / 1e12 if t1 > 0 else 0.0})
-        classes.sort(key=lambda c: -c['ms_per_launch'])
-        vj = numpy.empty_like(dm)
-        vk = numpy.empty_like(dm)
-        eng.check(L.mjk_memcpy_d2h(ctx, ptr(vj), d_vj, nbytes), 'd2h')
-        eng.check(L.mjk_memcpy_d2h(ctx, ptr(vk), d_vk, nbytes), 'd2h')
-        return {'t_total': t_total, 'lat': lat, 'classes': classes, 'stats': st, 'vj': vj, 'vk': vk,
-                'span_ms': span.value / max(nspan.value, 1)}
-
-    r64 = timed(0.0)
-    rmx = timed(args.fp32_tol) if args.fp32_tol > 0 else None
-    for p in (d_dm, d_vj, d_vk):
-        L.mjk_device_free(ctx, p)
-    # mixed-precision SCF energy
-    emx = None
-    if rmx is not None and e64 is not None and not args.omega:
-        mfm = rhf.RHF(mol, direct_scf_tol=tol, fp32_tol=args.fp32_tol)
-        mfm.conv_tol = 1e-10
-        emx = mfm.kernel()
-    t_total, lat, classes, st = r64['t_total'], r64['lat'], r64['classes'], r64['stats']
-    # the classes run concurrently on several streams: the dominant kernel is the
-    # one with the most algorithmic FLOPs; its duration is measured while it
-    # shares the GPU with the others
-    dom = max(classes, key=lambda c: c['flops'])
-    span_ms = r64['span_ms']
-    ms_step = t_total / args.steps * 1e3
-    builds = args.steps / t_total
-    kernel_ms = sum(c['ms_per_launch'] for c in classes)
-    total_flops = sum(c['flops'] for c in classes)
-    out = {
-        'metric': METRIC,
-        'value': builds,
-        'unit': 'JK builds/s',
-        'n_gpus': 1,
-        'steps': args.steps,
-        'warmup': args.warmup,
-        'ms_per_step': ms_step,
-        'higher_is_better': True,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': 'f64',
-        'data': 'synthetic geometry; converged RHF density from this engine',
-        'config': {'workload': desc + ', one J/K build of the converged RHF density, hermi=1, '
-                   'direct_scf_tol=1e-13', 'nao': nao, 'n_dm': 1, 'precision': 'fp64', 'omega': args.omega},
-        'quartets_per_s': st.quartets * builds,
-        'quartets_per_build': int(st.quartets),
-        'p50_ms': float(numpy.percentile(lat, 50) * 1e3),
-        'p99_ms': float(numpy.percentile(lat, 99) * 1e3),
-        'scf': scf_info,
-        'gpu_quartet_phase_ms_per_build': span_ms,
-        'sum_of_class_kernel_ms_per_build': kernel_ms,
-        'fp64_tflops_all_classes': total_flops / (span_ms * 1e-3) / 1e12,
-        # compute-bound against the FP64 peak (78.6 TFLOP/s is both the matrix and the
-        # vector FP64 peak of MI355X; the Rys kernels run on the vector ALU)
-        'roofline': {'bound': 'mfma', 'compute': 'fp64 vector ALU', 'kernel': dom['kernel'], 'class': dom['class'],
-                     'achieved': dom['tflops'], 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
-                     'frac': dom['tflops'] * 1e12 / FP64_PEAK,
-                     'traffic': traffic_bytes(dom['kernel']) if args.workload == 'c2' else None,
-                     'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch'],
-                     'build': {'what': 'all quartet-class kernels of one J/K build, concurrent on 4 streams',
-                               'achieved': total_flops / (span_ms * 1e-3) / 1e12,
-                               'frac': total_flops / (span_ms * 1e-3) / FP64_PEAK,
-                               'flops': total_flops, 'ms': span_ms}},
-        'classes': classes[:8],
-    }
-    if rmx is not None:
-        b_mx = args.steps / rmx['t_total']
-        out['mixed'] = {
-            'fp32_tol': args.fp32_tol,
-            'jk_builds_per_s': b_mx,
-            'speedup_vs_fp64': b_mx / builds,
-            'quartets_per_s': rmx['stats'].quartets * b_mx,
-            'fp32_quartet_fraction': rmx['stats'].quartets_fp32 / max(rmx['stats'].quartets, 1),
-            'p50_ms': float(numpy.percentile(rmx['lat'], 50) * 1e3),
-            'p99_ms': float(numpy.percentile(rmx['lat'], 99) * 1e3),
-            'max_abs_dJ': float(abs(rmx['vj'] - r64['vj']).max()),
-            'max_abs_dK': float(abs(rmx['vk'] - r64['vk']).max()),
-            'e_tot': emx,
-            'dE_vs_fp64': emx - e64 if emx is not None else None,
-        }
-    if not args.no_cpu:
-        with mol.with_range_coulomb(args.omega or None):
-            out['cpu_baseline'] = cpu_baseline(mol, dm, tol, args.cpu_seconds)
-    print(json.dumps(out))
+    return main_single(args)
 
 
 if __name__ == '__main__':

@@ -82,13 +82,13 @@ size_t mjk_ctx_hbm_used(const mjk_ctx* ctx);
  * agree either way to rounding. */
 int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes);
 
-/* No reference counterpart (a launch-overhead option): when enabled, a build
- * without statistics or profiling is captured once into a HIP graph and
- * replayed while its inputs, flags and buffers are unchanged.  Off by
- * default: a capture fails (the build returns MJK_EDEVICE) if another thread
- * of the process synchronises the device meanwhile (hipFree does, and so
- * does destroying another basis handle); enable it only where no other
- * thread frees device memory during builds. */
+/* No reference counterpart (a launch-overhead option): a build without
+ * statistics or profiling is captured once into a HIP graph and replayed
+ * while its inputs, flags and buffers are unchanged (one graph launch per
+ * build instead of one launch per kernel).  On by default; 0 runs every
+ * build eagerly.  Device memory that any thread of the process frees while
+ * a capture is in flight is released after the capture ends, so captures
+ * and frees in other threads may overlap. */
 int mjk_ctx_set_graphs(mjk_ctx* ctx, int enable);
 
 /* Upload one molecule's basis and build its shell pairs, Schwarz bounds
@@ -110,6 +110,12 @@ int mjk_basis_upload_batch(mjk_ctx* ctx, int nmol, const int* natm, const int* n
                            const int* const* atm, const int* const* bas,
                            const double* const* env, double cutoff, mjk_basis** out);
 void mjk_basis_destroy(mjk_basis* basis);
+
+/* Batched SCF (no reference counterpart): active[m] = 0 drops molecule m's
+ * quartets from the following builds of the handle at block level (its J
+ * and K come back zero); 1 keeps them.  All molecules start active.  The
+ * reference converges one molecule per SCF loop (pyscf/scf/hf.py:171-208). */
+int mjk_basis_set_active(mjk_ctx* ctx, mjk_basis* basis, const int* active);
 
 /* nao of molecule `mol` (spherical AOs). */
 int mjk_basis_nao(const mjk_basis* basis, int mol);
@@ -179,6 +185,14 @@ int mjk_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
  * get_ovlp/get_hcore (pyscf/scf/hf.py:315-345, pyscf/gto/mole.py:3474). */
 int mjk_int1e(int which, const int* atm, int natm, const int* bas, int nbas, const double* env,
               double* out);
+
+/* Cartesian -> real solid harmonic coefficients the engine uses for shells of
+ * angular momentum l <= 3: out[m * ncart(l) + c], m = 0..2l in PySCF's AO
+ * order (p: x, y, z; l >= 2: m = -l..l), Cartesian components c in the order
+ * lx = l..0, ly = l-lx..0, including the s and p angular factors
+ * 1/(2 sqrt(pi)) and sqrt(3/(4 pi)).  Replaces libcint's CINTc2s_ket_sph as
+ * used by pyscf.gto.cart2sph (pyscf/gto/mole.py:157-187); host only. */
+int mjk_c2s(int l, double* out);
 
 /* Rys roots x_i (= t_i^2) and weights of the interpolation table, evaluated
  * on the host (table validation).  n <= 9.  No reference counterpart (the

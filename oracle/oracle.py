@@ -14,6 +14,8 @@ Python side of the oracle:
   * ``rhf``          the RHF SCF loop of pyscf/scf/hf.py:47-242 with CDIIS
                      (pyscf/scf/diis.py:40-66) and the incremental Fock build
                      (pyscf/scf/hf.py:2416-2427); core-Hamiltonian initial guess
+  * ``set_roots``    per-T roots (default) or interpolated roots (full-size
+                     molecules)
   * ``boys_mp``      Boys function F_m(T) to 30 digits with mpmath (pins the
                      Rys roots: sum_i w_i x_i^m == F_m(T))
   * ``set_quadrature`` the t-integral of every primitive quartet by Rys
@@ -87,6 +89,13 @@ def hermi_triu(mat, hermi=1):
     elif hermi == 2:
         mat[..., iu[0], iu[1]] = -mat[..., iu[1], iu[0]]
     return mat
+
+
+def set_roots(mode):
+    """0: roots by the per-T Stieltjes procedure (default); 1: interpolated
+    from that procedure's values on a 1/32 grid in T (8-point Lagrange,
+    ~1e-14 relative; for full-size molecules)."""
+    lib().orc_set_roots(ctypes.c_int(mode))
 
 
 def set_quadrature(mode, panels=8):

@@ -94,6 +94,7 @@ struct JKArgs {
     int rprim;
     const int* pmol;        // molecule of every pair
     int nbucket;
+    const int* mol_active;  // per molecule: 0 skips its quartets (batched SCF; null: all active)
 };
 
 struct QCondArgs {

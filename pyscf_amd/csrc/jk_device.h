@@ -545,6 +545,7 @@ __global__ __launch_bounds__(64) void k_jk(const JKArgs a)
     c.p = t.x + (lane >> 3);
     c.q = t.z + kt * TILE + (lane & 7);
     c.mol = a.bt_mol[bt];
+    if (a.mol_active && !a.mol_active[c.mol]) return;   // block-uniform
     c.pvalid = c.p < t.y;
     c.qvalid = c.q < t.w;
     c.alive = c.pvalid && c.qvalid && !(a.diag && c.q > c.p);
@@ -693,6 +694,7 @@ __global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(con
     const int p = t.x + row;
     if (p >= t.y) return;
     const int mol = a.bt_mol[bt];
+    if (a.mol_active && !a.mol_active[mol]) return;   // block-uniform
     const double4 rA = a.P.rA[p];
     const double4 AB = a.P.rAB[p];
     const int pb0 = a.P.prim0[p], npb = a.P.nprim[p];
@@ -1077,6 +1079,7 @@ void k_jk_row(const JKArgs a)
     const int4 task = uload(a.rt + blockIdx.x);
     const int p = task.x;
     const int mol = uload(a.pmol + p);
+    if (a.mol_active && !uload(a.mol_active + mol)) return;   // block-uniform
     const int ld = uload(a.M.ld + mol);
     const long long base = uload(a.M.dbase + mol);
     const int krow = NROW * ld;

@@ -23,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -163,7 +164,7 @@ struct mjk_ctx {
     static constexpr int NAUX = 3;
     int nstreams = 4;                     // 1 + aux streams used, MJK_STREAMS
     int sched = 1;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
-    int graphs = 0;                       // replay plain builds as HIP graphs (mjk_ctx_set_graphs, MJK_GRAPHS)
+    int graphs = 1;                       // replay plain builds as HIP graphs (mjk_ctx_set_graphs, MJK_GRAPHS)
     int order = 1;                        // issue order within a stream: 0 by cost, 1 heavy classes first; MJK_ORDER
     double expcutoff = EXPCUTOFF_DEFAULT; // primitive pairs with exp(-ab/p AB^2) < e^-expcutoff dropped; MJK_EXPCUTOFF
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
@@ -204,6 +205,42 @@ int dalloc(mjk_ctx* ctx, size_t bytes, void** p)
     return MJK_OK;
 }
 
+// hipFree synchronises the whole device, and a synchronisation from any thread
+// while another thread captures a stream into a graph invalidates that
+// capture.  Captures therefore hold g_capture shared, and a free runs only
+// when it can take g_capture exclusively; otherwise the pointer waits in
+// g_deferred and is released by the next free or build that finds no capture
+// in flight.  (Budget accounting releases the bytes at once.)
+std::shared_mutex g_capture;
+std::mutex g_deferred_mu;
+std::vector<std::pair<int, void*>> g_deferred;   // {device, pointer}
+
+void free_now(int device, void* p)
+{
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != device) (void)hipSetDevice(device);
+    (void)hipFree(p);
+    if (cur != device) (void)hipSetDevice(cur);
+}
+
+// Frees what earlier calls deferred; caller holds g_capture exclusively.
+void drain_deferred_locked()
+{
+    std::vector<std::pair<int, void*>> todo;
+    {
+        std::lock_guard<std::mutex> lk(g_deferred_mu);
+        todo.swap(g_deferred);
+    }
+    for (auto& d : todo) free_now(d.first, d.second);
+}
+
+void drain_deferred()
+{
+    std::unique_lock<std::shared_mutex> lk(g_capture, std::try_to_lock);
+    if (lk.owns_lock()) drain_deferred_locked();
+}
+
 void dfree(mjk_ctx* ctx, void* p)
 {
     if (!p) return;
@@ -212,7 +249,14 @@ void dfree(mjk_ctx* ctx, void* p)
         ctx->used -= it->second;
         ctx->allocs.erase(it);
     }
-    (void)hipFree(p);
+    std::unique_lock<std::shared_mutex> lk(g_capture, std::try_to_lock);
+    if (!lk.owns_lock()) {
+        std::lock_guard<std::mutex> dl(g_deferred_mu);
+        g_deferred.push_back({ctx->device, p});
+        return;
+    }
+    drain_deferred_locked();
+    free_now(ctx->device, p);
 }
 
 template <typename T>
@@ -220,7 +264,12 @@ int upload(mjk_ctx* ctx, const std::vector<T>& v, T** d)
 {
     int rc = dalloc(ctx, v.size() * sizeof(T), (void**)d);
     if (rc) return rc;
-    if (!v.empty()) HIPCHK(ctx, hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!v.empty()) {
+        // on the context's (non-blocking) stream: no null-stream synchronisation
+        // that could meet another thread's graph capture
+        HIPCHK(ctx, hipMemcpyAsync(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return MJK_OK;
 }
 
@@ -252,6 +301,8 @@ struct mjk_basis {
     // the last plain build (no statistics, no profiling) as an instantiated HIP graph
     hipGraphExec_t gexec = nullptr;
     GraphKey gkey;
+    GraphKey last_key;          // key of the last plain build: a build is captured when its key repeats
+    bool has_last_key = false;
     std::vector<MolHost> mols;
     int maxld = 0, maxnao = 0, maxnsh = 0;
     long long slot_s = 0, slot_c = 0, cond_total = 0;
@@ -267,6 +318,7 @@ struct mjk_basis {
     double4 *rA = nullptr, *rAB = nullptr;
     std::vector<int> seg;            // [(class * nmol + mol) * NBUCKET + bucket] -> first pair
     int* d_pmol = nullptr;           // molecule of every pair
+    int* d_active = nullptr;         // per molecule: 0 skips its quartets (mjk_basis_set_active)
     std::vector<double> q;           // Schwarz bound per pair (sorted order)
     std::vector<int> h_shA, h_shB, h_nprim;
     std::vector<long long> pcost;    // prefix sums of h_nprim (row-kernel task sizes)
@@ -837,6 +889,8 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
                 for (int p = b->seg[((size_t)X * nmol + m) * NBUCKET]; p < b->seg[((size_t)X * nmol + m + 1) * NBUCKET]; ++p)
                     pmol[p] = m;
         if ((rc = track_upload(b, pmol, &b->d_pmol))) return rc;
+        std::vector<int> act(nmol, 1);
+        if ((rc = track_upload(b, act, &b->d_active))) return rc;
     }
 
     // per molecule / AO maps
@@ -963,8 +1017,11 @@ void mjk_ctx_destroy(mjk_ctx* ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto& kv : ctx->allocs) (void)hipFree(kv.first);
-    ctx->allocs.clear();
+    {
+        std::vector<void*> ps;
+        for (auto& kv : ctx->allocs) ps.push_back(kv.first);
+        for (void* p : ps) dfree(ctx, p);
+    }
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (int i = 0; i < mjk_ctx::NAUX; ++i) {
@@ -1063,6 +1120,19 @@ void mjk_basis_destroy(mjk_basis* b)
     delete b;
 }
 
+int mjk_basis_set_active(mjk_ctx* ctx, mjk_basis* b, const int* active)
+{
+    if (!ctx || !b || b->ctx != ctx || !active) return fail(ctx, MJK_EINVAL, "bad arguments to mjk_basis_set_active");
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<int> act(b->nmol);
+    for (int m = 0; m < b->nmol; ++m) act[m] = active[m] != 0;
+    // stream-ordered before the builds that follow (a replayed graph reads the same buffer)
+    HIPCHK(ctx, hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * b->nmol, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MJK_OK;
+}
+
 int mjk_basis_nao(const mjk_basis* b, int mol)
 {
     if (!b || mol < 0 || mol >= b->nmol) return MJK_EINVAL;
@@ -1104,6 +1174,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     std::lock_guard<std::recursive_mutex> lock(ctx->mu);
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    drain_deferred();
     const bool screen = direct_scf_tol > 0;
     const double cut = screen ? direct_scf_tol : 0.0;
     int rc;
@@ -1163,6 +1234,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.omega = b->omega;
     a.pmol = b->d_pmol;
     a.nbucket = NBUCKET;
+    a.mol_active = b->nmol > 1 ? b->d_active : nullptr;
     // classes, heaviest first, each on the least loaded of the streams (they all
     // add into the same J'/K' with atomics, so their order does not matter)
     const int nst = ctx->nstreams;
@@ -1254,8 +1326,18 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         key.cut = cut;
         key.fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
         key.row_lds = ctx->row_lds;
+        const bool repeat = b->has_last_key && b->last_key == key;
+        b->last_key = key;
+        b->has_last_key = true;
+        if (!(b->gexec && b->gkey == key) && !repeat) {
+            // a build seen once runs eagerly: one-shot requests never pay for a capture
+            drop_graph(b);
+            if ((rc = enqueue(false))) return rc;
+            return MJK_OK;
+        }
         if (!(b->gexec && b->gkey == key)) {
             drop_graph(b);
+            std::shared_lock<std::shared_mutex> cap(g_capture);   // no hipFree meanwhile
             HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             rc = enqueue(true);
             hipGraph_t g = nullptr;
@@ -1479,6 +1561,16 @@ int mjk_synchronize(mjk_ctx* ctx)
 {
     if (!ctx) return MJK_EINVAL;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MJK_OK;
+}
+
+int mjk_c2s(int l, double* out)
+{
+    if (l < 0 || l > C2S_MAXL || !out) return MJK_EINVAL;
+    // l <= 1: identity blocks here, the angular factors sit in the contraction coefficients
+    const double f = l == 0 ? C2S_S0 : l == 1 ? C2S_P1 : 1.0;
+    for (int m = 0; m < nsph(l); ++m)
+        for (int c = 0; c < ncart(l); ++c) out[m * ncart(l) + c] = f * C2S[l][m][c];
     return MJK_OK;
 }
 

@@ -47,6 +47,7 @@ def _declare(lib):
     lib.mjk_basis_destroy.argtypes = [c_void_p]
     lib.mjk_basis_destroy.restype = None
     lib.mjk_basis_nao.argtypes = [c_void_p, c_int]
+    lib.mjk_basis_set_active.argtypes = [c_void_p, c_void_p, c_void_p]
     lib.mjk_basis_slot_size.argtypes = [c_void_p]
     lib.mjk_basis_slot_size.restype = ctypes.c_longlong
     lib.mjk_basis_q_cond.argtypes = [c_void_p, c_void_p, c_int, c_void_p]
@@ -61,6 +62,7 @@ def _declare(lib):
     lib.mjk_mem_info.argtypes = [c_int, P(c_size_t), P(c_size_t)]
     lib.mjk_device_copy.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]
     lib.mjk_rys_roots.argtypes = [c_int, c_double, c_void_p, c_void_p]
+    lib.mjk_c2s.argtypes = [c_int, c_void_p]
     lib.mjk_profile.argtypes = [c_void_p, c_void_p, c_int]
     lib.mjk_profile_read.argtypes = [c_void_p, c_void_p, c_int] + [c_void_p] * 6
     lib.mjk_profile_kinds.argtypes = [c_void_p, c_void_p, c_int, c_void_p]

@@ -133,6 +133,15 @@ class Basis:
         except Exception:
             pass
 
+    def set_active(self, active):
+        """Batched builds: molecules with active[m] false are skipped at block
+        level by the following builds (their J and K come back zero)."""
+        a = numpy.ascontiguousarray(numpy.asarray(active, dtype=bool).astype(numpy.int32))
+        if a.size != len(self.naos):
+            raise ValueError('need one flag per molecule (%d), got %d' % (len(self.naos), a.size))
+        self.engine.check(self.engine.lib.mjk_basis_set_active(self.engine.handle, self.handle, _p(a)),
+                          'mjk_basis_set_active')
+
     def q_cond(self, mol=0):
         n = self.nbas[mol]
         out = numpy.zeros((n, n))

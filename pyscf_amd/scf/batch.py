@@ -2,9 +2,10 @@
 
 Every SCF iteration is ONE batched J/K build: one launch per quartet class
 covers the shell pairs of all molecules (the molecules' pairs sit back to
-back in the pair lists of ``mjk_basis_upload_batch``).  Molecules that have
-converged contribute a zero density difference, so the density screen
-(dm_cond = 0) drops all of their quartets from the launch.  Per-molecule
+back in the pair lists of ``mjk_basis_upload_batch``), and after the first
+build each iteration replays the build's HIP graph.  Molecules that have
+converged are marked inactive (``mjk_basis_set_active``): their blocks exit
+at once, so an iteration costs what its active molecules cost.  Per-molecule
 work (Fock assembly, CDIIS, generalized eigh) follows pyscf/scf/hf.py:47-242
 as in ``rhf.RHF``.
 """
@@ -30,7 +31,9 @@ class BatchRHF:
         self.converged = None
         self.cycles = 0
         self.build_times = []
-        self.build_quartets = []
+        self.build_active = []      # active molecules of every build
+        self.build_quartets = []    # evaluated quartets of every build (stats=True only)
+        self.stats = False
 
     def _offsets(self):
         naos = [m.nao_nr() for m in self.mols]
@@ -38,16 +41,26 @@ class BatchRHF:
         off[1:] = numpy.cumsum([n * n for n in naos])
         return naos, off
 
-    def get_jk_batch(self, dms):
-        """J and K for one density per molecule (list of (nao, nao))."""
+    def get_jk_batch(self, dms, active=None):
+        """J and K for one density per molecule (list of (nao, nao)); with
+        ``active`` (one flag per molecule) the inactive molecules are skipped
+        and get zero J and K."""
         naos, off = self._offsets()
+        if active is None:
+            active = numpy.ones(len(naos), dtype=bool)
+        active = numpy.asarray(active, dtype=bool)
+        if not numpy.array_equal(active, getattr(self, '_active_set', None)):
+            self.basis.set_active(active)
+            self._active_set = active.copy()
         flat = numpy.zeros((1, int(off[-1])))
-        for i, d in enumerate(dms):
-            flat[0, off[i]:off[i + 1]] = numpy.asarray(d).ravel()
+        for i in numpy.nonzero(active)[0]:
+            flat[0, off[i]:off[i + 1]] = numpy.asarray(dms[i]).ravel()
         t0 = time.perf_counter()
-        vj, vk = self.basis.get_jk(flat, 1, True, True, self.direct_scf_tol, self.fp32_tol, stats=True)
+        vj, vk = self.basis.get_jk(flat, 1, True, True, self.direct_scf_tol, self.fp32_tol, stats=self.stats)
         self.build_times.append(time.perf_counter() - t0)
-        self.build_quartets.append(self.basis.last_stats['quartets'])
+        self.build_active.append(int(active.sum()))
+        if self.stats:
+            self.build_quartets.append(self.basis.last_stats['quartets'])
         js = [vj[0, off[i]:off[i + 1]].reshape(n, n) for i, n in enumerate(naos)]
         ks = [vk[0, off[i]:off[i + 1]].reshape(n, n) for i, n in enumerate(naos)]
         return js, ks
@@ -78,8 +91,8 @@ class BatchRHF:
                 occ = drivers[i].get_occ(e)
                 new_dms[i] = drivers[i].make_rdm1(c, occ)
                 drivers[i].mo_coeff, drivers[i].mo_occ = c, occ
-            ddm = [new_dms[i] - dms[i] if active[i] else numpy.zeros_like(dms[i]) for i in range(len(mols))]
-            js, ks = self.get_jk_batch(ddm)
+            ddm = [new_dms[i] - dms[i] if active[i] else None for i in range(len(mols))]
+            js, ks = self.get_jk_batch(ddm, active)
             self.cycles = cycle + 1
             for i in numpy.nonzero(active)[0]:
                 vhf[i] = vhf[i] + js[i] - .5 * ks[i]

@@ -1,4 +1,5 @@
 """The CPU oracle against the reference's golden values and closed forms."""
+import ctypes
 import math
 
 import numpy
@@ -139,3 +140,40 @@ def test_planted_mistake_is_caught(oracle, pins, h2o_vdz):
     vj, vk = oracle.get_jk(h2o_vdz, dm, hermi=0)
     wrong = numpy.tril(vk) + numpy.tril(vk, -1).T
     assert abs(numpy.linalg.norm(wrong) - pins['pin2_h2o_ccpvdz_norm_K']['value']) > 1e-3
+
+
+@pytest.mark.parametrize('n', [1, 2, 3, 5, 7, 9])
+def test_interpolated_roots_match_boys_moments(oracle, n):
+    """The oracle's interpolated roots (set_roots(1), used for full-size
+    molecules) reproduce the Boys moments F_m(T), m < 2n, like the per-T
+    roots; T includes grid nodes, midpoints and the Hermite switch."""
+    rng = numpy.random.default_rng(100 + n)
+    Ts = numpy.concatenate([[0.0, 1 / 64, 1 / 32, 5.0 + 1 / 64, 36.0 + 6 * n - 1e-9],
+                            rng.uniform(0, 36.0 + 6 * n, 30)])
+    oracle.set_roots(1)
+    try:
+        for T in Ts:
+            x, w = oracle.rys_roots(n, float(T))   # per-T roots (orc_rys_roots)
+            xf = numpy.zeros(n)
+            wf = numpy.zeros(n)
+            oracle.lib().orc_rys_roots_fast(ctypes.c_int(n), ctypes.c_double(T), xf.ctypes.data_as(ctypes.c_void_p),
+                                            wf.ctypes.data_as(ctypes.c_void_p))
+            assert abs(xf - x).max() < 1e-13 and (abs(wf - w) / w).max() < 1e-13, (n, T)
+            for m in range(2 * n):
+                ref = float(oracle.boys_mp(m, T))
+                assert abs(numpy.dot(wf, xf**m) - ref) <= 2e-13 * ref, (n, T, m)
+    finally:
+        oracle.set_roots(0)
+
+
+def test_interpolated_roots_same_jk(oracle, h2o_vdz):
+    """J/K of H2O cc-pVDZ with the interpolated roots equal the per-T ones."""
+    numpy.random.seed(1)
+    dm = numpy.random.random((24, 24))
+    j0, k0 = oracle.get_jk(h2o_vdz, dm, hermi=0)
+    oracle.set_roots(1)
+    try:
+        j1, k1 = oracle.get_jk(h2o_vdz, dm, hermi=0)
+    finally:
+        oracle.set_roots(0)
+    assert abs(j1 - j0).max() < 1e-12 and abs(k1 - k0).max() < 1e-12
