@@ -7,6 +7,8 @@ Restates the reference SCF loop (pyscf/scf/hf.py:47-242):
   * aufbau occupation, D = 2 C_occ C_occ^T (pyscf/scf/hf.py:840);
   * incremental Fock build vhf = vhf_last + J(dD) - K(dD)/2 with screening
     at direct_scf_tol (pyscf/scf/hf.py:2416-2427);
+  * optional level shift of the virtual space, F + shift (S - S D/2 S),
+    applied after DIIS (pyscf/scf/hf.py:766-786, 1123-1124);
   * converged when |dE| < conv_tol and |g| < sqrt(conv_tol)
     (pyscf/scf/hf.py:194-197).
 The initial guess is the core Hamiltonian ('1e'); the reference defaults to
@@ -81,6 +83,7 @@ class RHF:
         self.direct_scf_tol = direct_scf_tol
         self.fp32_tol = fp32_tol
         self.diis_space = 8
+        self.level_shift = 0.0
         self.verbose = 0
         self._opt = None
         self.e_tot = 0.0
@@ -181,6 +184,8 @@ class RHF:
             dm_last, last_e = dm, e_tot
             fock = h1e + vhf
             fock = diis.update(s1e, dm, fock)
+            if abs(self.level_shift) > 1e-4:
+                fock = fock + self.level_shift * (s1e - s1e @ (dm * .5) @ s1e)
             mo_energy, mo_coeff = self.eig(fock, s1e)
             mo_occ = self.get_occ(mo_energy)
             dm = self.make_rdm1(mo_coeff, mo_occ)

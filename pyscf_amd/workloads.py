@@ -8,9 +8,10 @@ their textual definitions with fixed seeds.
                        molecule, numpy.random.default_rng(seed)
 * ``alkane``           all-trans CnH(2n+2) zig-zag, C-C 1.54, C-H 1.09
                        Angstrom, tetrahedral angles
-* ``random_chain``     seeded random walk of 3-12 heavy atoms (C/N/O),
-                       bonds 1.40-1.50 Angstrom, angles 105-115 degrees,
-                       uniform dihedrals, capped with H at 1.09 Angstrom
+* ``random_chain``     seeded self-avoiding random walk of 3-12 heavy
+                       atoms (C/N/O), bonds 1.40-1.50 Angstrom, angles
+                       105-115 degrees, uniform dihedrals, capped with H at
+                       1.09 Angstrom on tetrahedral sites
 """
 
 import math
@@ -84,48 +85,139 @@ def alkane(n=30, dcc=1.54, dch=1.09):
     return atoms_to_str(atoms)
 
 
-def random_chain(seed, nmin=3, nmax=12, bond=(1.40, 1.50), angle=(105.0, 115.0), dch=1.09):
-    """Random-walk chain of heavy atoms capped with hydrogens.
-
-    Heavy atoms: uniform over C/N/O; each heavy atom gets 4 - (neighbours)
-    valences filled with H (C), 3 - n (N), 2 - n (O), so the electron count
-    stays even; returns an atom string."""
-    rng = numpy.random.default_rng(seed)
-    n = int(rng.integers(nmin, nmax + 1))
-    elems = [str(rng.choice(['C', 'N', 'O'])) for _ in range(n)]
+def _chain_positions(rng, n, bond, angle, tries=200):
+    """Heavy-atom positions of one self-avoiding random walk: bond lengths
+    and angles uniform in their ranges, dihedrals uniform; a step that comes
+    closer than 2.2 Angstrom to an earlier atom, or 2.6 Angstrom to one at
+    least three bonds back (room for the hydrogens), is drawn again.
+    Returns None when a step finds no place in `tries` draws."""
     pos = [numpy.zeros(3)]
     if n > 1:
         pos.append(numpy.array([rng.uniform(*bond), 0.0, 0.0]))
     for i in range(2, n):
-        b = rng.uniform(*bond)
-        th = math.radians(rng.uniform(*angle))
-        ph = rng.uniform(0, 2 * math.pi)
         a, bb = pos[i - 2], pos[i - 1]
         u = _unit(bb - a)
         ref = numpy.array([0.0, 0.0, 1.0]) if abs(u[2]) < 0.9 else numpy.array([1.0, 0.0, 0.0])
         v = _unit(numpy.cross(u, ref))
         w = numpy.cross(u, v)
-        d = -math.cos(th) * u + math.sin(th) * (math.cos(ph) * v + math.sin(ph) * w)
-        pos.append(bb + b * d)
-    valence = {'C': 4, 'N': 3, 'O': 2}
-    atoms = [(e, tuple(p)) for e, p in zip(elems, pos)]
-    nH = 0
-    for i in range(n):
-        nb = [pos[j] for j in (i - 1, i + 1) if 0 <= j < n]
-        k = valence[elems[i]] - len(nb)
-        if k <= 0:
-            continue
-        if nb:
-            axis = -_unit(sum(_unit(x - pos[i]) for x in nb))
+        earlier = numpy.array(pos[:i - 1])
+        lim = numpy.where(numpy.arange(i - 1) <= i - 3, 2.6, 2.2)
+        for _ in range(tries):
+            b = rng.uniform(*bond)
+            th = math.radians(rng.uniform(*angle))
+            ph = rng.uniform(0, 2 * math.pi)
+            d = -math.cos(th) * u + math.sin(th) * (math.cos(ph) * v + math.sin(ph) * w)
+            cand = bb + b * d
+            if (numpy.linalg.norm(earlier - cand, axis=1) >= lim).all():
+                pos.append(cand)
+                break
         else:
-            axis = numpy.array([1.0, 0.0, 0.0])
-        ref = numpy.array([0.0, 0.0, 1.0]) if abs(axis[2]) < 0.9 else numpy.array([1.0, 0.0, 0.0])
-        v = _unit(numpy.cross(axis, ref))
-        w = numpy.cross(axis, v)
-        spread = 0.0 if k == 1 else math.radians(55.0)
-        for h in range(k):
-            phi = 2 * math.pi * h / k + rng.uniform(0, 0.2)
-            d = math.cos(spread) * axis + math.sin(spread) * (math.cos(phi) * v + math.sin(phi) * w)
-            atoms.append(('H', tuple(pos[i] + dch * _unit(d))))
-            nH += 1
-    return atoms_to_str(atoms)
+            return None
+    return pos
+
+
+def _cap_directions(rng, nbr_units, k):
+    """Unit vectors of k hydrogens on a heavy atom whose bonds to its heavy
+    neighbours point along nbr_units: tetrahedral sites away from those
+    bonds (two neighbours: the two sites off the heavy-atom plane, or the
+    bisector for one H; one neighbour: sites at 109.47 degrees from the
+    bond, 120 degrees apart, with a random twist)."""
+    if k <= 0:
+        return []
+    if len(nbr_units) == 2:
+        u1, u2 = nbr_units
+        a = -_unit(u1 + u2)
+        nrm = numpy.cross(u1, u2)
+        if numpy.linalg.norm(nrm) < 1e-8:      # collinear neighbours
+            ref = numpy.array([0.0, 0.0, 1.0]) if abs(a[2]) < 0.9 else numpy.array([1.0, 0.0, 0.0])
+            nrm = numpy.cross(a, ref)
+        nrm = _unit(nrm)
+        if k == 1:
+            return [a]
+        half = TETRA / 2
+        return [math.cos(half) * a + s * math.sin(half) * nrm for s in (1.0, -1.0)][:k]
+    if len(nbr_units) == 1:
+        axis = -nbr_units[0]
+    else:
+        axis = numpy.array([1.0, 0.0, 0.0])
+    ref = numpy.array([0.0, 0.0, 1.0]) if abs(axis[2]) < 0.9 else numpy.array([1.0, 0.0, 0.0])
+    v = _unit(numpy.cross(axis, ref))
+    w = numpy.cross(axis, v)
+    tilt = math.pi - TETRA
+    twist = rng.uniform(0, 2 * math.pi)
+    dirs = []
+    for h in range(k):
+        if not nbr_units and h == 0:
+            dirs.append(axis)
+            continue
+        phi = twist + 2 * math.pi * h / 3
+        dirs.append(math.cos(tilt) * axis + math.sin(tilt) * (math.cos(phi) * v + math.sin(phi) * w))
+    return dirs
+
+
+# closest approach allowed between atoms that are not bonded to each other
+MIN_SEPARATION = {('H', 'H'): 1.5, ('H', 'X'): 1.8, ('X', 'X'): 2.2}
+
+
+def _clash(atoms, bonded):
+    xyz = numpy.array([p for _, p in atoms])
+    kind = ['H' if s == 'H' else 'X' for s, _ in atoms]
+    d = numpy.linalg.norm(xyz[:, None] - xyz[None], axis=-1)
+    for i in range(len(atoms)):
+        for j in range(i):
+            if (j, i) in bonded:
+                continue
+            lim = MIN_SEPARATION[tuple(sorted((kind[i], kind[j])))]
+            if d[i, j] < lim:
+                return True
+    return False
+
+
+def random_chain(seed, nmin=3, nmax=12, bond=(1.40, 1.50), angle=(105.0, 115.0), dch=1.09, tries=1000):
+    """Self-avoiding random-walk chain of heavy atoms capped with hydrogens.
+
+    Heavy atoms: uniform over C/N/O; each heavy atom gets 4 - (neighbours)
+    valences filled with H (C), 3 - n (N), 2 - n (O) on tetrahedral sites,
+    so the electron count stays even.  Steps of the walk and the twist of a
+    chain end's hydrogens are drawn again while they bring two atoms that
+    are not bonded closer than MIN_SEPARATION (a walk that cannot continue
+    starts over), all from the one generator, so the chain is a function of
+    the seed alone.  Returns an atom string."""
+    rng = numpy.random.default_rng(seed)
+    n = int(rng.integers(nmin, nmax + 1))
+    elems = [str(rng.choice(['C', 'N', 'O'])) for _ in range(n)]
+    valence = {'C': 4, 'N': 3, 'O': 2}
+    lim_h = {'H': MIN_SEPARATION[('H', 'H')], 'X': MIN_SEPARATION[('H', 'X')]}
+    for _ in range(tries):
+        pos = _chain_positions(rng, n, bond, angle)
+        if pos is None:
+            continue
+        atoms = [(e, tuple(p)) for e, p in zip(elems, pos)]
+        bonded = {(i, i + 1) for i in range(n - 1)}
+        ok = True
+        for i in range(n):
+            nb = [j for j in (i - 1, i + 1) if 0 <= j < n]
+            units = [_unit(pos[j] - pos[i]) for j in nb]
+            k = valence[elems[i]] - len(nb)
+            if k <= 0:
+                continue
+            others = [(s_, numpy.array(p)) for idx, (s_, p) in enumerate(atoms) if idx != i]
+            for _twist in range(50):
+                hs = [pos[i] + dch * _unit(d) for d in _cap_directions(rng, units, k)]
+                if all(numpy.linalg.norm(h - p) >= lim_h['H' if s_ == 'H' else 'X']
+                       for h in hs for s_, p in others):
+                    break
+                if len(nb) == 2:        # fixed sites: no other twist to try
+                    hs = None
+                    break
+            else:
+                hs = None
+            if hs is None:
+                ok = False
+                break
+            for h in hs:
+                bonded.add((i, len(atoms)))
+                atoms.append(('H', tuple(h)))
+        if ok and not _clash(atoms, bonded):
+            return atoms_to_str(atoms)
+    raise RuntimeError('no self-avoiding chain for seed %r in %d draws' % (seed, tries))
