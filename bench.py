@@ -37,6 +37,7 @@ import ctypes
 import glob
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -49,6 +50,8 @@ METRIC = 'JK builds/s + ERI quartets/s, mixed vs FP64, at <=1e-8 Eh; p50/p99 lat
 FP64_PEAK = 78.6e12
 TOL = 1e-13
 _T0 = time.perf_counter()
+# host cores of this process, read before OpenMP binds the main thread to one of them
+NCORES = len(os.sched_getaffinity(0))
 
 
 def progress(msg):
@@ -109,6 +112,10 @@ def parse():
                     help='counter runs: two builds of the saved density, then a 64 MiB calibration copy')
     ap.add_argument('--pmc-summary', nargs=2, default=None, metavar=('FETCH_DIR', 'WRITE_DIR'),
                     help='HBM bytes per launch and kernel from the FETCH_SIZE and WRITE_SIZE runs')
+    ap.add_argument('--class-reps', type=int, default=20,
+                    help='c1-c3: timed launches of every quartet class alone (0: skip the per-class times)')
+    ap.add_argument('--cpu-child', default=None, metavar='NPZ',
+                    help='internal: the CPU side (oracle, CPU baseline) of a single-molecule run')
     return ap.parse_args()
 
 
@@ -137,7 +144,7 @@ def oracle_module():
     OpenMP runtime starts), with interpolated roots."""
     os.environ.setdefault('OMP_PROC_BIND', 'close')
     from oracle import oracle as O
-    O.set_threads(len(os.sched_getaffinity(0)))
+    O.set_threads(NCORES)
     O.set_roots(1)
     return O
 
@@ -155,7 +162,7 @@ def cpu_baseline(mol, dm, seconds):
     build: stride chosen for about `seconds` per run, 1 warm-up, median of 5,
     scaled to whole builds by the quartet count."""
     O = oracle_module()
-    ncores = len(os.sched_getaffinity(0))
+    ncores = NCORES
     opt = O.VHFOpt(mol, TOL)
     dms = dm[None]
     _, _, n_total = O.direct_full(mol, dms, opt, pair_stride=0)
@@ -178,107 +185,109 @@ def cpu_baseline(mol, dm, seconds):
                       '5 runs %.2f s, scaled by quartet count' % (stride, n_s, n_total, dt)}
 
 
-class DeviceBuild:
-    """The converged density resident in HBM and timed mjk_get_jk_device builds."""
-
-    def __init__(self, basis, dm):
-        from pyscf_amd.scf import _vhf  # noqa: F401
-        self.basis = basis
-        self.eng = basis.engine
-        self.L = self.eng.lib
-        self.ctx = self.eng.handle
-        self.dm = dm
-        self.nbytes = dm.nbytes
-        self.bufs = [ctypes.c_void_p() for _ in range(3)]
-        for p in self.bufs:
-            self.eng.check(self.L.mjk_device_alloc(self.ctx, self.nbytes, ctypes.byref(p)), 'device_alloc')
-        self.eng.check(self.L.mjk_memcpy_h2d(self.ctx, self.bufs[0], self.ptr(dm), self.nbytes), 'h2d')
-
-    @staticmethod
-    def ptr(a):
-        return a.ctypes.data_as(ctypes.c_void_p)
-
-    def free(self):
-        for p in self.bufs:
-            self.L.mjk_device_free(self.ctx, p)
-
-    def build(self, fp32_tol, stats=None):
-        d_dm, d_vj, d_vk = self.bufs
-        rc = self.L.mjk_get_jk_device(self.ctx, self.basis.handle, d_dm, 1, 1, 1, 1, TOL, fp32_tol, d_vj, d_vk,
-                                      ctypes.byref(stats) if stats is not None else None)
-        self.eng.check(rc, 'mjk_get_jk_device')
-
-    def sync(self):
-        self.eng.check(self.L.mjk_synchronize(self.ctx), 'sync')
-
-    def read_profile(self):
-        L, ctx, h, p = self.L, self.ctx, self.basis.handle, self.ptr
-        maxc = 64
-        cls = numpy.zeros(4 * maxc, dtype=numpy.int32)
-        ms, fl = numpy.zeros(maxc), numpy.zeros(maxc)
-        nq, npq, nl = (numpy.zeros(maxc, dtype=numpy.int64) for _ in range(3))
-        k = L.mjk_profile_read(ctx, h, maxc, p(cls), p(ms), p(nq), p(npq), p(fl), p(nl))
-        kinds = numpy.zeros(maxc, dtype=numpy.int32)
-        L.mjk_profile_kinds(ctx, h, maxc, p(kinds))
-        span, nspan = ctypes.c_double(), ctypes.c_longlong()
-        self.eng.check(L.mjk_profile_span(ctx, h, ctypes.byref(span), ctypes.byref(nspan)), 'profile_span')
-        out = {}
-        for i in range(k):
-            key = tuple(int(x) for x in cls[4 * i:4 * i + 4])
-            out[key] = {'ms': ms[i], 'launches': int(nl[i]), 'nq': int(nq[i]), 'flops': fl[i], 'kind': int(kinds[i])}
-        return out, span.value, nspan.value
-
-    def timed(self, fp32_tol, steps, warmup):
-        """One counted build (per-class work), warm-up builds, then `steps`
-        synchronised builds; HIP events time every class launch of all of
-        them (profiling mode 2 keeps no tallies)."""
-        from pyscf_amd import lib
-        L, ctx, h = self.L, self.ctx, self.basis.handle
-        st = lib.mjk_stats()
-        self.eng.check(L.mjk_profile(ctx, h, 1), 'profile')
-        self.build(fp32_tol, st)                  # tile lists for this cutoff, work buffers, tallies
-        counted, _, _ = self.read_profile()
-        self.eng.check(L.mjk_profile(ctx, h, 2), 'profile')
-        for _ in range(warmup):
-            self.build(fp32_tol)
-        self.sync()
-        warm, _, _ = self.read_profile()
-        self.eng.check(L.mjk_profile(ctx, h, 2), 'profile')
-        lat = []
-        self.sync()
-        t_start = time.perf_counter()
-        for _ in range(steps):
-            t = time.perf_counter()
-            self.build(fp32_tol)
-            self.sync()
-            lat.append(time.perf_counter() - t)
-        t_total = time.perf_counter() - t_start
-        prof, span, nspan = self.read_profile()
-        L.mjk_profile(ctx, h, 0)
-        classes = []
-        for key, r in prof.items():
-            c = counted.get(key, {'nq': 0, 'flops': 0.0, 'ms': 0.0, 'launches': 0})
-            w = warm.get(key, {'ms': 0.0, 'launches': 0})
-            t1 = r['ms'] / r['launches']
-            n_all = r['launches'] + c['launches'] + w['launches']
-            classes.append({'class': '(%d%d|%d%d)' % key, 'kernel': kernel_name(key, r['kind'], self.basis_omega()),
-                            'ms_per_launch': t1,
-                            'ms_per_launch_all_launches': (r['ms'] + c['ms'] + w['ms']) / max(n_all, 1),
-                            'launches_all': n_all, 'quartets': c['nq'], 'flops': c['flops'],
-                            'tflops': c['flops'] / (t1 * 1e-3) / 1e12 if t1 > 0 else 0.0})
-        classes.sort(key=lambda c: -c['ms_per_launch'])
-        vj, vk = numpy.empty_like(self.dm), numpy.empty_like(self.dm)
-        self.eng.check(L.mjk_memcpy_d2h(ctx, self.ptr(vj), self.bufs[1], self.nbytes), 'd2h')
-        self.eng.check(L.mjk_memcpy_d2h(ctx, self.ptr(vk), self.bufs[2], self.nbytes), 'd2h')
-        return {'t_total': t_total, 'lat': lat, 'classes': classes, 'stats': st, 'vj': vj, 'vk': vk,
-                'span_ms': span / max(nspan, 1), 'steps': steps}
-
-    def basis_omega(self):
-        return bool(getattr(self.basis, 'omega', 0.0))
+def cpu_child(args):
+    """CPU side of a single-molecule benchmark, in a process of its own that
+    never opens the GPU: the oracle build of the density the GPU used (max
+    |dJ|, |dK| of the GPU results against it) and the CPU baseline.  Reads the
+    .npz the GPU process wrote; prints one JSON line."""
+    data = numpy.load(args.cpu_child, allow_pickle=False)
+    name = str(data['workload'])
+    omega = float(data['omega'])
+    mol, _ = workload(name)
+    dm = numpy.ascontiguousarray(data['dm'])
+    out = {}
+    if not args.no_oracle:
+        progress('%s: oracle build of the same density (CPU process)' % name)
+        with mol.with_range_coulomb(omega or None):
+            j0, k0, t_orc = oracle_jk(mol, dm)
+        acc = {'reference': 'oracle (CPU restatement of the reference direct J/K), same density and screening',
+               'oracle_build_s': t_orc,
+               'fp64_max_abs_dJ': float(abs(data['vj64'] - j0).max()),
+               'fp64_max_abs_dK': float(abs(data['vk64'] - k0).max())}
+        if 'vjmx' in data:
+            acc['mixed_max_abs_dJ'] = float(abs(data['vjmx'] - j0).max())
+            acc['mixed_max_abs_dK'] = float(abs(data['vkmx'] - k0).max())
+        out['accuracy'] = acc
+    if not args.no_cpu:
+        progress('%s: CPU baseline (CPU process)' % name)
+        with mol.with_range_coulomb(omega or None):
+            out['cpu_baseline'] = cpu_baseline(mol, dm, args.cpu_seconds)
+    print(json.dumps(out))
 
 
-def single_molecule(args, name, steps, warmup, with_cpu, with_oracle, fp32_tol):
-    """SCF, timed FP64 (and mixed) builds, oracle accuracy, CPU baseline."""
+def run_cpu_child(args, name, omega, dm, r64, rmx, with_oracle, with_cpu):
+    """Writes the density and the GPU J/K, runs the CPU child, returns its JSON."""
+    if not (with_oracle or with_cpu):
+        return {}
+    os.makedirs(os.path.join(ROOT, 'gpurun_out'), exist_ok=True)
+    path = os.path.join(ROOT, 'gpurun_out', 'bench_%s_%d.npz' % (name, os.getpid()))
+    arrays = {'workload': numpy.array(name), 'omega': numpy.array(omega), 'dm': dm,
+              'vj64': r64['vj'], 'vk64': r64['vk']}
+    if rmx is not None:
+        arrays['vjmx'], arrays['vkmx'] = rmx['vj'], rmx['vk']
+    numpy.savez(path, **arrays)
+    cmd = [sys.executable, os.path.abspath(__file__), '--cpu-child', path, '--cpu-seconds', str(args.cpu_seconds)]
+    if not with_oracle:
+        cmd.append('--no-oracle')
+    if not with_cpu:
+        cmd.append('--no-cpu')
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, check=True)
+    os.remove(path)
+    return json.loads(res.stdout.decode().strip().splitlines()[-1])
+
+
+def timed_builds(basis, fp32_tol, steps, warmup):
+    """Warm-up builds, then `steps` plain builds of the resident density (each
+    a graph replay, synchronised, J and K left in HBM): wall time per build."""
+    for _ in range(warmup):
+        basis.get_jk_resident(1, 1, True, True, TOL, fp32_tol)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        basis.get_jk_resident(1, 1, True, True, TOL, fp32_tol)
+        lat.append(time.perf_counter() - t)
+    return time.perf_counter() - t_start, lat
+
+
+def class_times(basis, fp32_tol, reps, omega):
+    """Each quartet class alone: builds restricted to it (class filter) with
+    statistics, whose HIP events on the build stream bracket exactly that
+    class's kernel launch (the class runs on the build stream when it is the
+    only one); one untimed build first, then the mean over `reps`."""
+    out = []
+    for k, l4, nblocks, kern in basis.classes():
+        if nblocks == 0:
+            continue
+        basis.set_class_filter(k)
+        st = basis.get_jk_resident(1, 1, True, True, TOL, fp32_tol, stats=True)
+        ms = []
+        for _ in range(reps):
+            ms.append(basis.get_jk_resident(1, 1, True, True, TOL, fp32_tol, stats=True)['ms_quartets'])
+        t1 = float(numpy.mean(ms))
+        out.append({'class': '(%d%d|%d%d)' % l4, 'kernel': kernel_name(l4, kern, bool(omega)),
+                    'ms_per_launch': t1, 'launches': reps, 'quartets': int(st['quartets']),
+                    'quartets_fp32': int(st['quartets_fp32']), 'flops': st['flops'],
+                    'tflops': st['flops'] / (t1 * 1e-3) / 1e12 if t1 > 0 else 0.0})
+    basis.set_class_filter(-1)
+    out.sort(key=lambda c: -c['flops'])
+    return out
+
+
+def gpu_leg(basis, dm, fp32_tol, steps, warmup, class_reps, omega):
+    """Plain timed builds, one counted build (work and GPU phase times) and the
+    per-class timings of one precision; J and K of the density."""
+    vj, vk = basis.get_jk(dm[None], 1, True, True, TOL, fp32_tol)      # uploads the density
+    t_total, lat = timed_builds(basis, fp32_tol, steps, warmup)
+    st = basis.get_jk_resident(1, 1, True, True, TOL, fp32_tol, stats=True)
+    classes = class_times(basis, fp32_tol, class_reps, omega) if class_reps > 0 else []
+    return {'t_total': t_total, 'lat': lat, 'stats': st, 'classes': classes,
+            'vj': vj.reshape(dm.shape), 'vk': vk.reshape(dm.shape)}
+
+
+def single_molecule(args, name, steps, warmup, with_cpu, with_oracle, fp32_tol, class_reps):
+    """SCF, timed FP64 (and mixed) builds, per-class kernel times, then the
+    oracle accuracy and the CPU baseline in a CPU-only child process."""
     from pyscf_amd.scf import _vhf, rhf
     mol, desc = workload(name)
     nao = mol.nao_nr()
@@ -302,54 +311,31 @@ def single_molecule(args, name, steps, warmup, with_cpu, with_oracle, fp32_tol):
     if args.omega:
         with mol.with_range_coulomb(args.omega):
             opt = _vhf.VHFOpt(mol, direct_scf_tol=TOL)
-        opt.basis.omega = args.omega
-    dev = DeviceBuild(opt.basis, dm)
-    progress('%s: %d timed FP64 builds' % (name, steps))
-    r64 = dev.timed(0.0, steps, warmup)
+    progress('%s: %d timed FP64 builds, per-class times' % (name, steps))
+    r64 = gpu_leg(opt.basis, dm, 0.0, steps, warmup, class_reps, args.omega)
     rmx = None
     if fp32_tol > 0:
         progress('%s: %d timed mixed builds' % (name, steps))
-        rmx = dev.timed(fp32_tol, steps, warmup)
-    dev.free()
+        rmx = gpu_leg(opt.basis, dm, fp32_tol, steps, warmup, 0, args.omega)
     emx = None
     if rmx is not None and e64 is not None and not args.omega:
         progress('%s: mixed-precision RHF' % name)
         mfm = rhf.RHF(mol, direct_scf_tol=TOL, fp32_tol=fp32_tol)
         mfm.conv_tol = 1e-10
         emx = mfm.kernel()
-    accuracy = None
-    if with_oracle:
-        progress('%s: oracle build of the same density' % name)
-        with mol.with_range_coulomb(args.omega or None):
-            j0, k0, t_orc = oracle_jk(mol, dm)
-        accuracy = {'reference': 'oracle (CPU restatement of the reference direct J/K), same density and '
-                                 'screening', 'oracle_build_s': t_orc,
-                    'fp64_max_abs_dJ': float(abs(r64['vj'] - j0).max()),
-                    'fp64_max_abs_dK': float(abs(r64['vk'] - k0).max())}
-        if rmx is not None:
-            accuracy['mixed_max_abs_dJ'] = float(abs(rmx['vj'] - j0).max())
-            accuracy['mixed_max_abs_dK'] = float(abs(rmx['vk'] - k0).max())
-    cpu = None
-    if with_cpu:
-        progress('%s: CPU baseline' % name)
-        with mol.with_range_coulomb(args.omega or None):
-            cpu = cpu_baseline(mol, dm, args.cpu_seconds)
+    cpu = run_cpu_child(args, name, args.omega, dm, r64, rmx, with_oracle, with_cpu)
     return {'mol': mol, 'desc': desc, 'nao': nao, 'scf': scf_info, 'e64': e64, 'emx': emx,
-            'r64': r64, 'rmx': rmx, 'accuracy': accuracy, 'cpu': cpu}
+            'r64': r64, 'rmx': rmx, 'accuracy': cpu.get('accuracy'), 'cpu': cpu.get('cpu_baseline')}
 
 
 def main_single(args):
     res = single_molecule(args, args.workload, args.steps, args.warmup, not args.no_cpu, not args.no_oracle,
-                          args.fp32_tol)
+                          args.fp32_tol, args.class_reps)
     r64, rmx = res['r64'], res['rmx']
     t_total, lat, classes, st = r64['t_total'], r64['lat'], r64['classes'], r64['stats']
-    # the classes run concurrently on several streams: the dominant kernel is the
-    # one with the most algorithmic FLOPs; its duration is measured while it
-    # shares the GPU with the others
-    dom = max(classes, key=lambda c: c['flops'])
-    span_ms = r64['span_ms']
     builds = args.steps / t_total
-    total_flops = sum(c['flops'] for c in classes)
+    quartet_ms = st['ms_quartets']
+    total_flops = st['flops']
     out = {
         'metric': METRIC, 'value': builds, 'unit': 'JK builds/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': t_total / args.steps * 1e3, 'higher_is_better': True,
@@ -358,31 +344,34 @@ def main_single(args):
         'config': {'workload': res['desc'] + ', one J/K build of the converged RHF density, hermi=1, '
                    'direct_scf_tol=1e-13', 'nao': res['nao'], 'n_dm': 1, 'precision': 'fp64',
                    'omega': args.omega},
-        'quartets_per_s': st.quartets * builds, 'quartets_per_build': int(st.quartets),
+        'quartets_per_s': st['quartets'] * builds, 'quartets_per_build': int(st['quartets']),
         'p50_ms': float(numpy.percentile(lat, 50) * 1e3), 'p99_ms': float(numpy.percentile(lat, 99) * 1e3),
         'scf': res['scf'],
-        'gpu_quartet_phase_ms_per_build': span_ms,
-        'fp64_tflops_all_classes': total_flops / (span_ms * 1e-3) / 1e12,
-        'roofline': {'bound': 'valu', 'compute': 'FP64 vector ALU (no MFMA: Rys recurrences)',
-                     'kernel': dom['kernel'], 'class': dom['class'],
-                     'achieved': dom['tflops'], 'peak': FP64_PEAK / 1e12, 'unit': 'TFLOP/s',
-                     'frac': dom['tflops'] * 1e12 / FP64_PEAK,
-                     'traffic': traffic_bytes(dom['kernel'], args.workload),
-                     'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch'],
-                     'ms_per_launch_all_launches': dom['ms_per_launch_all_launches'],
-                     'launches_all': dom['launches_all'],
-                     'build': {'what': 'all quartet-class kernels of one J/K build, concurrent on 4 streams',
-                               'achieved': total_flops / (span_ms * 1e-3) / 1e12,
-                               'frac': total_flops / (span_ms * 1e-3) / FP64_PEAK,
-                               'flops': total_flops, 'ms': span_ms}},
-        'classes': classes[:10],
+        'counted_build': {'what': 'one build with statistics (eager, HIP events on the build stream)',
+                          'ms_quartet_phase': quartet_ms, 'ms_transforms': st['ms_transform'],
+                          'fp64_tflops_quartet_phase': total_flops / (quartet_ms * 1e-3) / 1e12,
+                          'frac_of_fp64_peak': total_flops / (quartet_ms * 1e-3) / FP64_PEAK,
+                          'flops': total_flops},
     }
+    if classes:
+        dom = classes[0]
+        out['roofline'] = {
+            'bound': 'valu', 'compute': 'FP64 vector ALU (no MFMA: Rys recurrences)',
+            'kernel': dom['kernel'], 'class': dom['class'], 'achieved': dom['tflops'], 'peak': FP64_PEAK / 1e12,
+            'unit': 'TFLOP/s', 'frac': dom['tflops'] * 1e12 / FP64_PEAK,
+            'traffic': traffic_bytes(dom['kernel'], args.workload),
+            'flops_per_launch': dom['flops'], 'ms_per_launch': dom['ms_per_launch'],
+            'timing': 'the class alone on the build stream (class filter), HIP events around its launch, '
+                      'mean of %d launches' % dom['launches']}
+        out['classes'] = classes[:12]
+        out['class_sum_ms'] = sum(c['ms_per_launch'] for c in classes)
     if rmx is not None:
         b_mx = args.steps / rmx['t_total']
+        smx = rmx['stats']
         out['mixed'] = {
             'fp32_tol': args.fp32_tol, 'jk_builds_per_s': b_mx, 'speedup_vs_fp64': b_mx / builds,
-            'quartets_per_s': rmx['stats'].quartets * b_mx,
-            'fp32_quartet_fraction': rmx['stats'].quartets_fp32 / max(rmx['stats'].quartets, 1),
+            'quartets_per_s': smx['quartets'] * b_mx,
+            'fp32_quartet_fraction': smx['quartets_fp32'] / max(smx['quartets'], 1),
             'p50_ms': float(numpy.percentile(rmx['lat'], 50) * 1e3),
             'p99_ms': float(numpy.percentile(rmx['lat'], 99) * 1e3),
             'max_abs_dJ_vs_fp64_gpu': float(abs(rmx['vj'] - r64['vj']).max()),
@@ -395,12 +384,12 @@ def main_single(args):
         out['cpu_baseline'] = res['cpu']
     if args.workload == 'c3' and not args.no_secondary and not args.dm_in and not args.omega:
         progress('secondary: (H2O)16 def2-SVP')
-        sec = single_molecule(args, 'c2', args.steps, args.warmup, False, not args.no_oracle, 0.0)
+        sec = single_molecule(args, 'c2', args.steps, args.warmup, False, not args.no_oracle, 0.0, 0)
         s64 = sec['r64']
         out['secondary'] = {'workload': sec['desc'], 'jk_builds_per_s': args.steps / s64['t_total'],
                             'ms_per_build': s64['t_total'] / args.steps * 1e3,
                             'p50_ms': float(numpy.percentile(s64['lat'], 50) * 1e3),
-                            'quartets_per_build': int(s64['stats'].quartets),
+                            'quartets_per_build': int(s64['stats']['quartets']),
                             'scf': sec['scf'], 'accuracy': sec['accuracy']}
     print(json.dumps(out))
 
@@ -586,6 +575,8 @@ def main():
         print('bench.py runs on one GPU only: the J/K engine has no multi-GPU path; '
               'got --gpus %d' % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.cpu_child:
+        return cpu_child(args)
     if args.pmc_summary:
         return pmc_summary(args)
     if args.pmc_run:

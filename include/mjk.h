@@ -144,6 +144,32 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n
                       int with_j, int with_k, double direct_scf_tol, double fp32_tol,
                       double* d_vj, double* d_vk, mjk_stats* stats);
 
+/* Measurement and batching helpers with no reference counterpart.
+ *
+ * mjk_get_jk_resident: the J/K build of the densities that the last
+ * mjk_get_jk call on this basis uploaded (still resident in the basis's
+ * device buffers), with J and K left in device memory -- the build alone,
+ * without the host<->device copies of mjk_get_jk, replayed as a HIP graph
+ * like any plain build.  Synchronous.  MJK_EINVAL when no density of n_dm
+ * slots is resident.
+ * mjk_basis_fetch_jk: copies the resident J and/or K (either may be NULL)
+ * of n_dm slots to the host.
+ * mjk_basis_set_class_filter: cls >= 0 restricts the following builds of
+ * the handle to quartet class cls (its index in the basis's class list for
+ * the current cutoff), -1 restores all classes.  J and K are linear in the
+ * quartets, so the filtered builds of all classes add up to the full build.
+ * mjk_basis_nclasses / mjk_basis_class: the class list; l4 = (la lb lc ld),
+ * nblocks = launch size (0: no surviving task), kernel = the kernel a build
+ * of n_dm densities with this hermi runs for it (0 tile, 1 row,
+ * 2 cooperative). */
+int mjk_get_jk_resident(mjk_ctx* ctx, mjk_basis* basis, int n_dm, int hermi, int with_j, int with_k,
+                        double direct_scf_tol, double fp32_tol, mjk_stats* stats);
+int mjk_basis_fetch_jk(mjk_ctx* ctx, mjk_basis* basis, int n_dm, double* vj, double* vk);
+int mjk_basis_set_class_filter(mjk_ctx* ctx, mjk_basis* basis, int cls);
+int mjk_basis_nclasses(const mjk_basis* basis);
+int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* basis, int k, int n_dm, int hermi, int* l4,
+                    long long* nblocks, int* kernel);
+
 /* Per-quartet-class kernel profiling with HIP events on the launch stream
  * (the classes of a build run concurrently on up to four streams, so the
  * per-class times of one build overlap).

@@ -281,7 +281,7 @@ struct GraphKey {
     const void* vj = nullptr;
     const void* vk = nullptr;
     const void* bufs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    int n_dm = 0, hermi = 0, with_j = 0, with_k = 0, nstreams = 0;
+    int n_dm = 0, hermi = 0, with_j = 0, with_k = 0, nstreams = 0, class_filter = -1;
     double cut = 0, fp32_tol = 0;
     size_t row_lds = 0;
     bool operator==(const GraphKey& o) const
@@ -289,6 +289,7 @@ struct GraphKey {
         for (int i = 0; i < 5; ++i)
             if (bufs[i] != o.bufs[i]) return false;
         return dms == o.dms && vj == o.vj && vk == o.vk && n_dm == o.n_dm && hermi == o.hermi &&
+               class_filter == o.class_filter &&
                with_j == o.with_j && with_k == o.with_k && nstreams == o.nstreams && cut == o.cut &&
                fp32_tol == o.fp32_tol && row_lds == o.row_lds;
     }
@@ -319,6 +320,7 @@ struct mjk_basis {
     std::vector<int> seg;            // [(class * nmol + mol) * NBUCKET + bucket] -> first pair
     int* d_pmol = nullptr;           // molecule of every pair
     int* d_active = nullptr;         // per molecule: 0 skips its quartets (mjk_basis_set_active)
+    int class_filter = -1;           // >= 0: builds run only this quartet class (mjk_basis_set_class_filter)
     std::vector<double> q;           // Schwarz bound per pair (sorted order)
     std::vector<int> h_shA, h_shB, h_nprim;
     std::vector<long long> pcost;    // prefix sums of h_nprim (row-kernel task sizes)
@@ -1253,6 +1255,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         double load[1 + mjk_ctx::NAUX] = {0, 0, 0, 0};
         for (size_t c : corder) {
             if (b->qcls[c].nblocks == 0) continue;
+            if (b->class_filter >= 0 && (int)c != b->class_filter) continue;
             const int si = (int)(std::min_element(load, load + nst) - load);
             load[si] += cost(b->qcls[c]);
             cstream[c] = si;
@@ -1326,6 +1329,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         key.cut = cut;
         key.fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
         key.row_lds = ctx->row_lds;
+        key.class_filter = b->class_filter;
         const bool repeat = b->has_last_key && b->last_key == key;
         b->last_key = key;
         b->has_last_key = true;
@@ -1371,6 +1375,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
                                   hipMemcpyDeviceToHost));
         for (size_t c = 0; c < b->qcls.size(); ++c) {
             if (b->qcls[c].nblocks == 0) continue;
+            if (b->class_filter >= 0 && (int)c != b->class_filter) continue;
             float ms = 0;
             HIPCHK(ctx, hipEventElapsedTime(&ms, b->pev[2 * c], b->pev[2 * c + 1]));
             b->prof_ms[c] += ms;
@@ -1436,6 +1441,71 @@ int mjk_get_jk(mjk_ctx* ctx, mjk_basis* b, const double* dms, int n_dm, int herm
     if (stats)
         stats->ms_total =
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return MJK_OK;
+}
+
+int mjk_get_jk_resident(mjk_ctx* ctx, mjk_basis* b, int n_dm, int hermi, int with_j, int with_k,
+                        double direct_scf_tol, double fp32_tol, mjk_stats* stats)
+{
+    if (!ctx || !b || b->ctx != ctx || n_dm <= 0) return fail(ctx, MJK_EINVAL, "bad arguments to mjk_get_jk_resident");
+    std::lock_guard<std::recursive_mutex> outer(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const long long n = (long long)n_dm * b->slot_s;
+    if (!b->Dsph || b->cap_ds < n)
+        return fail(ctx, MJK_EINVAL, "no resident density of %d slots: mjk_get_jk uploads one", n_dm);
+    int rc;
+    if (with_j && (rc = ensure(ctx, &b->Vj, b->cap_vj, n))) return rc;
+    if (with_k && (rc = ensure(ctx, &b->Vk, b->cap_vk, n))) return rc;
+    rc = mjk_get_jk_device(ctx, b, b->Dsph, n_dm, hermi, with_j, with_k, direct_scf_tol, fp32_tol,
+                           with_j ? b->Vj : nullptr, with_k ? b->Vk : nullptr, stats);
+    if (rc) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MJK_OK;
+}
+
+int mjk_basis_fetch_jk(mjk_ctx* ctx, mjk_basis* b, int n_dm, double* vj, double* vk)
+{
+    if (!ctx || !b || b->ctx != ctx || n_dm <= 0) return fail(ctx, MJK_EINVAL, "bad arguments to mjk_basis_fetch_jk");
+    std::lock_guard<std::recursive_mutex> outer(ctx->mu);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const long long n = (long long)n_dm * b->slot_s;
+    if ((vj && (!b->Vj || b->cap_vj < n)) || (vk && (!b->Vk || b->cap_vk < n)))
+        return fail(ctx, MJK_EINVAL, "no resident J/K of %d slots", n_dm);
+    if (vj) HIPCHK(ctx, hipMemcpyAsync(vj, b->Vj, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (vk) HIPCHK(ctx, hipMemcpyAsync(vk, b->Vk, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MJK_OK;
+}
+
+int mjk_basis_set_class_filter(mjk_ctx* ctx, mjk_basis* b, int cls)
+{
+    if (!ctx || !b || b->ctx != ctx) return fail(ctx, MJK_EINVAL, "context/basis mismatch");
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    if (cls < -1 || cls >= (int)b->qcls.size())
+        return fail(ctx, MJK_EINVAL, "class %d out of range (-1 .. %d)", cls, (int)b->qcls.size() - 1);
+    b->class_filter = cls;
+    return MJK_OK;
+}
+
+int mjk_basis_nclasses(const mjk_basis* b)
+{
+    return b ? (int)b->qcls.size() : MJK_EINVAL;
+}
+
+int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* b, int k, int n_dm, int hermi, int* l4, long long* nblocks,
+                    int* kernel)
+{
+    if (!ctx || !b || b->ctx != ctx || k < 0 || k >= (int)b->qcls.size() || !l4 || !nblocks || !kernel)
+        return fail(ctx, MJK_EINVAL, "bad arguments to mjk_basis_class");
+    const QClass& qc = b->qcls[k];
+    l4[0] = qc.la, l4[1] = qc.lb, l4[2] = qc.lc, l4[3] = qc.ld;
+    *nblocks = qc.nblocks;
+    // the choice the build makes for J and K of n_dm densities
+    const int n_j = n_dm, n_k = hermi == 1 ? n_dm : 2 * n_dm;
+    const size_t shmem = sizeof(double) * ((size_t)n_k * (ncart(qc.la) + ncart(qc.lb)) * b->maxld +
+                                           (size_t)n_j * ncart(qc.la) * ncart(qc.lb));
+    const bool row = shmem <= ctx->row_lds && qc.nrt > 0 && row_kernel_available(qc.la, qc.lb, qc.lc, qc.ld);
+    *kernel = row ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
     return MJK_OK;
 }
 

@@ -169,6 +169,46 @@ class Basis:
         return vj, vk
 
 
+    def get_jk_resident(self, n_dm, hermi, with_j, with_k, direct_scf_tol, fp32_tol=0.0, stats=False):
+        """The build of the densities the last get_jk uploaded, J and K left
+        on the device (no host copies); returns the statistics with stats."""
+        st = _lib.mjk_stats()
+        rc = self.engine.lib.mjk_get_jk_resident(self.engine.handle, self.handle, int(n_dm), int(hermi),
+                                                 int(bool(with_j)), int(bool(with_k)), float(direct_scf_tol),
+                                                 float(fp32_tol), ctypes.byref(st) if stats else None)
+        self.engine.check(rc, 'mjk_get_jk_resident')
+        if stats:
+            self.last_stats = st.as_dict()
+            return self.last_stats
+        return None
+
+    def fetch_jk(self, n_dm, with_j=True, with_k=True):
+        """Resident J and K of the last build, as (n_dm, slot) arrays."""
+        vj = numpy.empty((n_dm, self.slot)) if with_j else None
+        vk = numpy.empty((n_dm, self.slot)) if with_k else None
+        self.engine.check(self.engine.lib.mjk_basis_fetch_jk(self.engine.handle, self.handle, int(n_dm),
+                                                             _p(vj), _p(vk)), 'mjk_basis_fetch_jk')
+        return vj, vk
+
+    def set_class_filter(self, cls):
+        """Following builds run quartet class cls only (-1: all classes)."""
+        self.engine.check(self.engine.lib.mjk_basis_set_class_filter(self.engine.handle, self.handle, int(cls)),
+                          'mjk_basis_set_class_filter')
+
+    def classes(self, n_dm=1, hermi=1):
+        """[(index, (la, lb, lc, ld), nblocks, kernel)] of the class list;
+        kernel 0 tile, 1 row, 2 cooperative."""
+        L = self.engine.lib
+        out = []
+        l4 = numpy.zeros(4, dtype=numpy.int32)
+        nb, kern = ctypes.c_longlong(), ctypes.c_int()
+        for k in range(L.mjk_basis_nclasses(self.handle)):
+            self.engine.check(L.mjk_basis_class(self.engine.handle, self.handle, k, n_dm, hermi, _p(l4),
+                                                ctypes.byref(nb), ctypes.byref(kern)), 'mjk_basis_class')
+            out.append((k, tuple(int(x) for x in l4), int(nb.value), int(kern.value)))
+        return out
+
+
 class VHFOpt:
     """Screening context (counterpart of _VHFOpt, pyscf/scf/_vhf.py:151-264).
 
