@@ -59,13 +59,16 @@ def progress(msg):
     print('[bench %.0fs] %s' % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
 
 
-def kernel_name(cls, kind, rsh=False):
+def kernel_name(cls, kind, rsh=False, ctr=(1, 1, 1, 1)):
     """Demangled name (as rocprofv3 prints it, less the argument list) of the
-    kernel a quartet class ran on: 0 tile, 1 row, 2 cooperative; classes cut
-    into slices run one kernel per slice and are named by slice 0.  rsh: the
-    row-kernel variant of the range-separated operators."""
+    kernel a quartet class ran on: 0 tile, 1 row, 2 cooperative, 3 merged row
+    kernel (contraction counts ctr); classes cut into slices run one kernel
+    per slice and are named by slice 0.  rsh: the row-kernel variant of the
+    range-separated operators."""
     from pyscf_amd.build import jk_split
     la, lb, lc, ld = cls
+    if kind == 3:
+        return 'mjk::k_jk_row_gc<%d, %d, %d, %d, %d, %d, %d, %d>' % (tuple(cls) + tuple(ctr))
     if kind == 2:
         return 'mjk::k_jk_coop<%d, %d, %d, %d>' % cls
     if kind == 1:
@@ -256,8 +259,8 @@ def class_times(basis, fp32_tol, reps, omega):
     class's kernel launch (the class runs on the build stream when it is the
     only one); one untimed build first, then the mean over `reps`."""
     out = []
-    for k, l4, nblocks, kern in basis.classes():
-        if nblocks == 0:
+    for k, l4, ctr, nblocks, kern in basis.classes():
+        if nblocks == 0 or kern < 0:     # no task, or the other representation of its quartets runs
             continue
         basis.set_class_filter(k)
         st = basis.get_jk_resident(1, 1, True, True, TOL, fp32_tol, stats=True)
@@ -265,7 +268,8 @@ def class_times(basis, fp32_tol, reps, omega):
         for _ in range(reps):
             ms.append(basis.get_jk_resident(1, 1, True, True, TOL, fp32_tol, stats=True)['ms_quartets'])
         t1 = float(numpy.mean(ms))
-        out.append({'class': '(%d%d|%d%d)' % l4, 'kernel': kernel_name(l4, kern, bool(omega)),
+        label = '(%d%d|%d%d)' % l4 if ctr == (1, 1, 1, 1) else '(%d%d|%d%d) ctr %d%d%d%d' % (l4 + ctr)
+        out.append({'class': label, 'kernel': kernel_name(l4, kern, bool(omega), ctr),
                     'ms_per_launch': t1, 'launches': reps, 'quartets': int(st['quartets']),
                     'quartets_fp32': int(st['quartets_fp32']), 'flops': st['flops'],
                     'tflops': st['flops'] / (t1 * 1e-3) / 1e12 if t1 > 0 else 0.0})

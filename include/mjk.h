@@ -159,15 +159,17 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n
  * the current cutoff), -1 restores all classes.  J and K are linear in the
  * quartets, so the filtered builds of all classes add up to the full build.
  * mjk_basis_nclasses / mjk_basis_class: the class list; l4 = (la lb lc ld),
- * nblocks = launch size (0: no surviving task), kernel = the kernel a build
- * of n_dm densities with this hermi runs for it (0 tile, 1 row,
- * 2 cooperative). */
+ * ctr4 = contraction counts of the four shells (2 for a generally contracted
+ * s shell kept whole, 1 otherwise), nblocks = launch size (0: no surviving
+ * task), kernel = the kernel a build of n_dm densities with this hermi runs
+ * for it (0 tile, 1 row, 2 cooperative, 3 merged row kernel; -1 the class
+ * does not run: the other representation of its quartets does). */
 int mjk_get_jk_resident(mjk_ctx* ctx, mjk_basis* basis, int n_dm, int hermi, int with_j, int with_k,
                         double direct_scf_tol, double fp32_tol, mjk_stats* stats);
 int mjk_basis_fetch_jk(mjk_ctx* ctx, mjk_basis* basis, int n_dm, double* vj, double* vk);
 int mjk_basis_set_class_filter(mjk_ctx* ctx, mjk_basis* basis, int cls);
 int mjk_basis_nclasses(const mjk_basis* basis);
-int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* basis, int k, int n_dm, int hermi, int* l4,
+int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* basis, int k, int n_dm, int hermi, int* l4, int* ctr4,
                     long long* nblocks, int* kernel);
 
 /* Per-quartet-class kernel profiling with HIP events on the launch stream

@@ -1,9 +1,18 @@
 // Data layout shared by the host driver and the HIP kernels.
 //
 // Shell pairs live in HBM as structure-of-arrays, grouped by pair class
-// (la >= lb) and, inside a class, sorted by descending Schwarz bound.
-// Every primitive pair (exponent sum p, centre P, folded prefactor K, 1/p)
-// is stored once; a quartet task only reads two pair records.
+// (la >= lb) and pair group, and, inside a group, sorted by descending
+// Schwarz bound.  Every primitive pair (exponent sum p, centre P, folded
+// prefactor K, 1/p) is stored once; a quartet task only reads two pair
+// records.
+//
+// Generally contracted s shells with two contractions sharing their
+// primitives (cc-pVXZ cores) appear twice: as two segmented shells in
+// "split" pairs, and as one two-contraction unit in "merged" pairs, whose
+// primitive pairs carry the products of the contraction coefficients of
+// every contraction combination (ppW) so that one primitive loop serves all
+// combinations.  A quartet class runs either its split or its merged pair
+// groups, never both.
 //
 // Densities and outputs are kept in a Cartesian AO arena: for molecule m of
 // a batch, the block [dbase[m], dbase[m] + ld[m]^2) of every slot holds its
@@ -30,12 +39,16 @@ __host__ __device__ constexpr int pair_class(int la, int lb) { return la * (la +
 struct PairSet {
     const double4* ppP;    // per primitive pair {Px, Py, Pz, p}
     const double2* ppK;    // per primitive pair {K, 1/p}; K = ca cb exp(-ab/p AB^2) sqrt(2) pi^(5/4) / p
+                           // (merged pairs: without ca cb, which are in ppW)
+    const double4* ppW;    // merged pairs: ca[ci] cb[cj] of contraction combination ci * CB + cj
     const int* prim0;      // first primitive pair of each shell pair
     const int* nprim;      // primitive pairs of each shell pair
     const double4* rA;     // {Ax, Ay, Az, Schwarz q}
     const double4* rAB;    // {ABx, ABy, ABz, 0}  (A - B)
-    const int* shA;        // shell index of A within its molecule (dm_cond rows)
+    const int* shA;        // segmented shell index of A within its molecule (first contraction)
     const int* shB;
+    const int* oshA;       // shell index of A in the molecule's input basis (dm_cond rows)
+    const int* oshB;
     const int* cartA;      // Cartesian AO offset of A within its molecule
     const int* cartB;
     // Ket-side copy of the primitive pairs, transposed per (class, molecule,
@@ -44,6 +57,7 @@ struct PairSet {
     // read consecutive addresses.
     const double4* tP;
     const double2* tK;
+    const double4* tW;     // transposed ppW
     const int* tpos;
     const int* tstride;
 };
@@ -52,7 +66,7 @@ struct MolSet {
     const long long* dbase;   // arena offset of the molecule's Cartesian block
     const int* ld;            // ncart of the molecule
     const long long* cbase;   // offset of the molecule's dm_cond block
-    const int* nsh;           // shells of the molecule
+    const int* nsh;           // shells of the molecule's input basis (dm_cond dimension)
 };
 
 // One launch = one quartet class over a list of bra tiles.  A bra tile is
@@ -76,7 +90,7 @@ struct JKArgs {
     int ncopy;              // private accumulator copies (one per XCD); the kernel picks XCC_ID % ncopy
     long long jcopy;        // element stride between copies of J'
     long long kcopy;        // element stride between copies of K'
-    const double* dm_cond;  // per molecule (nsh x nsh)
+    const double* dm_cond;  // per molecule (nsh x nsh) over the input basis's shells
     double cutoff;
     int screen;
     double fp32_tol;        // > 0: waves whose quartets all have q_ij q_kl dmax < fp32_tol run in FP32

@@ -154,7 +154,7 @@ __device__ __forceinline__ double quartet_dmax(const JKArgs& a, int mol, int p, 
 {
     const double* __restrict__ dc = a.dm_cond + a.M.cbase[mol];
     const int ns = a.M.nsh[mol];
-    const int i = a.P.shA[p], j = a.P.shB[p], k = a.P.shA[q], l = a.P.shB[q];
+    const int i = a.P.oshA[p], j = a.P.oshB[p], k = a.P.oshA[q], l = a.P.oshB[q];
     // J terms only for J builds, K terms only for K builds (the J-only and
     // K-only predicates, pyscf/lib/vhf/optimizer.c:119-167)
     double d = 0.0;
@@ -1213,6 +1213,419 @@ int launch_jk_row(int nblocks, const JKArgs& a, size_t shmem, hipStream_t st)
     return (int)hipGetLastError();
 }
 
+
+
+// ---------------------------------------------------------------------------
+// Row kernel over merged pair groups.  A merged pair holds a generally
+// contracted s shell as one unit of two contractions sharing their
+// primitives; a pair group has CA x CB contraction combinations (CA, CB in
+// {1, 2}).  One primitive-quartet loop computes the primitive integral block
+// once and adds it, weighted by the coefficient products of each combination,
+// into the NIJ NKL contracted blocks of the quartet.  Contraction ci of a unit
+// sits ci ncart(l) Cartesian functions after its first one, so the bra's K'
+// rows are CA NA + CB NB consecutive rows of the arena.  No slices, no
+// range-separated variant: the classes that use it are small.
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD>
+struct GcShape {
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int NF = NA * NB * NC * ND;
+    static constexpr int NIJ = CA * CB, NKL = CC * CD, NCOMB = NIJ * NKL;
+    static constexpr int NRA = CA * NA, NRB = CB * NB;   // bra K' rows of A and of B
+};
+
+// Coefficient products of a pair's combinations: a plain pair (N = 1) has
+// them folded into its K; a merged pair's are in its W record.
+template <int N>
+__device__ __forceinline__ void combo_weights(const double4& w, double (&out)[N])
+{
+    if constexpr (N == 1) {
+        out[0] = 1.0;
+    } else {
+        out[0] = w.x;
+        out[1] = w.y;
+        if constexpr (N > 2) {
+            out[2] = w.z;
+            out[3] = w.w;
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, typename R>
+__device__ __forceinline__ void eri_gc(const JKArgs& a, const QuartetCtx& c,
+                                       double (&gd)[GcShape<LA, LB, LC, LD, CA, CB, CC, CD>::NCOMB]
+                                                   [GcShape<LA, LB, LC, LD, CA, CB, CC, CD>::NF])
+{
+    using G = GcShape<LA, LB, LC, LD, CA, CB, CC, CD>;
+    constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+    constexpr int NF = G::NF, NIJ = G::NIJ, NKL = G::NKL;
+    R gout[NIJ * NKL][NF];
+#pragma unroll
+    for (int k = 0; k < NIJ * NKL; ++k)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) gout[k][f] = R(0);
+    for (int ib = 0; ib < c.npb; ++ib) {
+        const double4 Pp = uload(a.P.ppP + c.pb0 + ib);
+        const double2 Kp = uload(a.P.ppK + c.pb0 + ib);
+        double wp[NIJ];
+        combo_weights<NIJ>(NIJ > 1 ? uload(a.P.ppW + c.pb0 + ib) : make_double4(1, 0, 0, 0), wp);
+        for (int ik = 0; ik < c.npk; ++ik) {
+            double T;
+            const int t0 = c.tq + ik * c.ts;
+            const PrimQuartet m = prim_quartet(Pp, Kp, a.P.tP[t0], a.P.tK[t0], c, T);
+            double wq[NKL];
+            combo_weights<NKL>(NKL > 1 ? a.P.tW[t0] : make_double4(1, 0, 0, 0), wq);
+            R tmp[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) tmp[f] = R(0);
+            const typename RysFor<NR, LA + LB + LC + LD>::type rys(a.rys, T);
+#pragma unroll 1
+            for (int r = 0; r < NR; ++r) {
+                double rx, rw;
+                rys.root(r, rx, rw);
+                add_root<LA, LB, LC, LD, 1, 0, R>(m, rx, rw, c, tmp);
+            }
+#pragma unroll
+            for (int i = 0; i < NIJ; ++i)
+#pragma unroll
+                for (int k = 0; k < NKL; ++k) {
+                    const R w = R(wp[i] * wq[k]);
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) gout[i * NKL + k][f] = fma(w, tmp[f], gout[i * NKL + k][f]);
+                }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NIJ * NKL; ++k)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) gd[k][f] = double(gout[k][f]) * c.fac;
+}
+
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, typename R>
+__device__ __forceinline__ void row_digest_gc(const JKArgs& a, const QuartetCtx& c, double* kr, double* jr,
+                                              int krow, long long base, int ld, int oa, int ob, int cp)
+{
+    using G = GcShape<LA, LB, LC, LD, CA, CB, CC, CD>;
+    constexpr int NA = G::NA, NB = G::NB, NC = G::NC, ND = G::ND, NF = G::NF;
+    constexpr int NKL = G::NKL, NRA = G::NRA, NRB = G::NRB;
+    double gd[G::NCOMB][NF];
+    eri_gc<LA, LB, LC, LD, CA, CB, CC, CD, R>(a, c, gd);
+    const int oc = c.qvalid ? a.P.cartA[c.q] : 0, od = c.qvalid ? a.P.cartB[c.q] : 0;
+    const int lane = threadIdx.x & 63;
+#define GV(ci, cj, ck, cl, ia, ib2, ic, id)                                                         \
+    gd[((ci) * CB + (cj)) * NKL + (ck) * CD + (cl)][(((ia) * NB + (ib2)) * NC + (ic)) * ND + (id)]
+    for (int s = 0; s < a.n_j; ++s) {
+        const double* __restrict__ D = a.Dj + s * a.slot + base;
+        double* V = a.J + cp * a.jcopy + s * a.slot + base;
+        // J'[c,d] += sum_ab v Ds[a,b]  (per quartet and ket combination, global)
+        double dab[NRA][NRB];
+#pragma unroll
+        for (int i = 0; i < NRA; ++i)
+#pragma unroll
+            for (int j = 0; j < NRB; ++j) dab[i][j] = D[(oa + i) * ld + ob + j];
+        if (c.alive) {
+#pragma unroll
+            for (int ck = 0; ck < CC; ++ck)
+#pragma unroll
+            for (int cl = 0; cl < CD; ++cl)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id) {
+                double acc = 0.0;
+#pragma unroll
+                for (int ci = 0; ci < CA; ++ci)
+#pragma unroll
+                for (int cj = 0; cj < CB; ++cj)
+#pragma unroll
+                for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+                for (int ib2 = 0; ib2 < NB; ++ib2)
+                    acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dab[ci * NA + ia][cj * NB + ib2], acc);
+                atomic_add(V + (oc + ck * NC + ic) * ld + od + cl * ND + id, acc);
+            }
+        }
+        // J'[a,b] += sum_cd v Ds[c,d]  (summed over the wave, then into LDS)
+        double dcd[CC * NC][CD * ND];
+#pragma unroll
+        for (int i = 0; i < CC * NC; ++i)
+#pragma unroll
+            for (int j = 0; j < CD * ND; ++j) dcd[i][j] = c.alive ? D[(oc + i) * ld + od + j] : 0.0;
+#pragma unroll
+        for (int ci = 0; ci < CA; ++ci)
+#pragma unroll
+        for (int cj = 0; cj < CB; ++cj)
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+        for (int ib2 = 0; ib2 < NB; ++ib2) {
+            double acc = 0.0;
+#pragma unroll
+            for (int ck = 0; ck < CC; ++ck)
+#pragma unroll
+            for (int cl = 0; cl < CD; ++cl)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < ND; ++id)
+                acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dcd[ck * NC + ic][cl * ND + id], acc);
+            acc = tile_sum<1>(acc);
+            acc = tile_sum<8>(acc);
+            if (lane == 0 && acc != 0.0) lds_add(jr + s * NRA * NRB + (ci * NA + ia) * NRB + cj * NB + ib2, acc);
+        }
+    }
+    if (!c.alive) return;
+    for (int s = 0; s < a.n_k; ++s) {
+        const double* __restrict__ D = a.Dk + s * a.slot + base;
+        double* KA = kr + s * krow;              // rows a (all contractions of A): K'[a, :]
+        double* KB = KA + NRA * ld;              // rows b: K'[b, :]
+        double dbc[NRB][CC * NC], dac[NRA][CC * NC], dbd[NRB][CD * ND], dad[NRA][CD * ND];
+#pragma unroll
+        for (int i = 0; i < NRB; ++i)
+#pragma unroll
+            for (int j = 0; j < CC * NC; ++j) dbc[i][j] = D[(ob + i) * ld + oc + j];
+#pragma unroll
+        for (int i = 0; i < NRA; ++i)
+#pragma unroll
+            for (int j = 0; j < CC * NC; ++j) dac[i][j] = D[(oa + i) * ld + oc + j];
+#pragma unroll
+        for (int i = 0; i < NRB; ++i)
+#pragma unroll
+            for (int j = 0; j < CD * ND; ++j) dbd[i][j] = D[(ob + i) * ld + od + j];
+#pragma unroll
+        for (int i = 0; i < NRA; ++i)
+#pragma unroll
+            for (int j = 0; j < CD * ND; ++j) dad[i][j] = D[(oa + i) * ld + od + j];
+        // K'[a,d] += sum_bc v D[b,c]
+#pragma unroll
+        for (int ci = 0; ci < CA; ++ci)
+#pragma unroll
+        for (int cl = 0; cl < CD; ++cl)
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) {
+            double acc = 0.0;
+#pragma unroll
+            for (int cj = 0; cj < CB; ++cj)
+#pragma unroll
+            for (int ck = 0; ck < CC; ++ck)
+#pragma unroll
+            for (int ib2 = 0; ib2 < NB; ++ib2)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+                acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dbc[cj * NB + ib2][ck * NC + ic], acc);
+            lds_add(KA + (ci * NA + ia) * ld + od + cl * ND + id, acc);
+        }
+        // K'[b,d] += sum_ac v D[a,c]
+#pragma unroll
+        for (int cj = 0; cj < CB; ++cj)
+#pragma unroll
+        for (int cl = 0; cl < CD; ++cl)
+#pragma unroll
+        for (int ib2 = 0; ib2 < NB; ++ib2)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) {
+            double acc = 0.0;
+#pragma unroll
+            for (int ci = 0; ci < CA; ++ci)
+#pragma unroll
+            for (int ck = 0; ck < CC; ++ck)
+#pragma unroll
+            for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ic = 0; ic < NC; ++ic)
+                acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dac[ci * NA + ia][ck * NC + ic], acc);
+            lds_add(KB + (cj * NB + ib2) * ld + od + cl * ND + id, acc);
+        }
+        // K'[a,c] += sum_bd v D[b,d]
+#pragma unroll
+        for (int ci = 0; ci < CA; ++ci)
+#pragma unroll
+        for (int ck = 0; ck < CC; ++ck)
+#pragma unroll
+        for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic) {
+            double acc = 0.0;
+#pragma unroll
+            for (int cj = 0; cj < CB; ++cj)
+#pragma unroll
+            for (int cl = 0; cl < CD; ++cl)
+#pragma unroll
+            for (int ib2 = 0; ib2 < NB; ++ib2)
+#pragma unroll
+            for (int id = 0; id < ND; ++id)
+                acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dbd[cj * NB + ib2][cl * ND + id], acc);
+            lds_add(KA + (ci * NA + ia) * ld + oc + ck * NC + ic, acc);
+        }
+        // K'[b,c] += sum_ad v D[a,d]
+#pragma unroll
+        for (int cj = 0; cj < CB; ++cj)
+#pragma unroll
+        for (int ck = 0; ck < CC; ++ck)
+#pragma unroll
+        for (int ib2 = 0; ib2 < NB; ++ib2)
+#pragma unroll
+        for (int ic = 0; ic < NC; ++ic) {
+            double acc = 0.0;
+#pragma unroll
+            for (int ci = 0; ci < CA; ++ci)
+#pragma unroll
+            for (int cl = 0; cl < CD; ++cl)
+#pragma unroll
+            for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int id = 0; id < ND; ++id)
+                acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dad[ci * NA + ia][cl * ND + id], acc);
+            lds_add(KB + (cj * NB + ib2) * ld + oc + ck * NC + ic, acc);
+        }
+    }
+#undef GV
+}
+
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD>
+__global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
+{
+    using G = GcShape<LA, LB, LC, LD, CA, CB, CC, CD>;
+    constexpr int NRA = G::NRA, NRB = G::NRB, NROW = NRA + NRB;
+    extern __shared__ double lds_rows[];
+    __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
+    __shared__ int s_next;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int4 task = uload(a.rt + blockIdx.x);
+    const int p = task.x;
+    const int mol = uload(a.pmol + p);
+    if (a.mol_active && !uload(a.mol_active + mol)) return;   // block-uniform
+    const int ld = uload(a.M.ld + mol);
+    const long long base = uload(a.M.dbase + mol);
+    const int krow = NROW * ld;
+    double* kr = lds_rows;
+    double* jr = lds_rows + a.n_k * krow;
+    const int nz = a.n_k * krow + a.n_j * NRA * NRB;
+    for (int e = tid; e < nz; e += 256) lds_rows[e] = 0.0;
+    const double qp = uload(a.P.rA + p).w;
+    if (tid < a.nbucket) {
+        const int2 r = a.rr[(long long)task.w * a.nbucket + tid];
+        s_start[tid] = r.x;
+        s_pre[tid + 1] = r.y - r.x;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        s_pre[0] = 0;
+        for (int k = 1; k <= a.nbucket; ++k) s_pre[k] += s_pre[k - 1];
+        s_next = task.y;
+    }
+    __syncthreads();
+    const int iend = min(s_pre[a.nbucket], task.z);
+
+    QuartetCtx c;
+    c.p = p;
+    c.mol = mol;
+    c.pvalid = true;
+    c.rA = uload(a.P.rA + p);
+    c.AB = uload(a.P.rAB + p);
+    const int pbo = uload(a.rtp + blockIdx.x);
+    c.pb0 = uload(a.P.prim0 + p) + pbo;
+    c.npb = min(a.rprim, uload(a.P.nprim + p) - pbo);
+    // a merged quartet counts as its NCOMB quartets of segmented shells, once
+    const unsigned long long quartet_tally = pbo == 0 ? G::NCOMB : 0;
+    const bool diagA = uload(a.P.shA + p) == uload(a.P.shB + p);
+    const int oa = uload(a.P.cartA + p), ob = uload(a.P.cartB + p);
+    const int cp = xcc_copy(a.ncopy);
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    for (;;) {
+        int i0 = 0;
+        if (lane == 0) i0 = atomicAdd(&s_next, 64);
+        i0 = __shfl(i0, 0);
+        if (i0 >= iend) break;   // wave-uniform
+        const int idx = i0 + lane;
+        c.qvalid = idx < iend;
+        c.q = 0;
+        if (c.qvalid) {
+            int lo = 0, hi = a.nbucket;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= idx) lo = mid; else hi = mid;
+            }
+            c.q = s_start[lo] + idx - s_pre[lo];
+        }
+        c.alive = c.qvalid;
+        c.rC = c.CD = make_double4(0, 0, 0, 0);
+        c.pk0 = c.npk = 0;
+        c.tq = c.ts = 0;
+        bool small = true;
+        if (c.alive) {
+            c.rC = a.P.rA[c.q];
+            const double qpq = qp * c.rC.w;
+            if (a.screen || a.fp32_tol > 0) {
+                const double dmax = quartet_dmax(a, mol, p, c.q);
+                if (a.screen) c.alive = qpq > a.cutoff && dmax * qpq > a.cutoff;
+                small = dmax * qpq < a.fp32_tol;
+            } else {
+                small = false;
+            }
+        }
+        c.fac = 0.0;
+        if (c.alive) {
+            c.pk0 = a.P.prim0[c.q];
+            c.npk = a.P.nprim[c.q];
+            c.tq = a.P.tpos[c.q];
+            c.ts = a.P.tstride[c.q];
+            c.CD = a.P.rAB[c.q];
+            double f = diagA ? 0.5 : 1.0;
+            if (a.P.shA[c.q] == a.P.shB[c.q]) f *= 0.5;
+            if (c.q == p && a.diag) f *= 0.5;
+            c.fac = f;
+        }
+        if (!__ballot(c.alive)) continue;   // wave-uniform
+        const bool use32 = a.fp32_tol > 0 && __ballot(c.alive && !small) == 0;
+        if (c.alive) {
+            cnt[use32 ? 2 : 0] += quartet_tally;
+            cnt[use32 ? 3 : 1] += (unsigned long long)(c.npb * c.npk);
+        }
+        if (use32) row_digest_gc<LA, LB, LC, LD, CA, CB, CC, CD, float>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
+        else row_digest_gc<LA, LB, LC, LD, CA, CB, CC, CD, double>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
+    }
+    __syncthreads();
+    // flush: rows r < NRA are A's rows oa + r, the others B's rows ob + r - NRA
+    for (int e = tid; e < a.n_k * krow; e += 256) {
+        const double v = kr[e];
+        if (v == 0.0) continue;
+        const int s = e / krow;
+        const int r = (e - s * krow) / ld;
+        const int col = e - s * krow - r * ld;
+        const int row = r < NRA ? oa + r : ob + (r - NRA);
+        atomic_add(a.K + cp * a.kcopy + s * a.slot + base + (long long)row * ld + col, v);
+    }
+    for (int e = tid; e < a.n_j * NRA * NRB; e += 256) {
+        const double v = jr[e];
+        if (v == 0.0) continue;
+        const int s = e / (NRA * NRB);
+        const int ra = (e / NRB) % NRA, rb = e % NRB;
+        atomic_add(a.J + cp * a.jcopy + s * a.slot + base + (long long)(oa + ra) * ld + ob + rb, v);
+    }
+    if (a.counters) {   // one atomic per counter and block
+        __shared__ unsigned long long s_cnt[4];
+        if (tid < 4) s_cnt[tid] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            unsigned long long v = cnt[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0 && v) atomicAdd(s_cnt + k, v);
+        }
+        __syncthreads();
+        if (tid < 4 && s_cnt[tid]) atomicAdd(a.counters + tid, s_cnt[tid]);
+    }
+}
+
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD>
+int launch_jk_row_gc(int nblocks, const JKArgs& a, size_t shmem, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_jk_row_gc<LA, LB, LC, LD, CA, CB, CC, CD>), dim3(nblocks), dim3(256), shmem, st, a);
+    return (int)hipGetLastError();
+}
 
 }  // namespace mjk
 

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <shared_mutex>
 #include <numeric>
@@ -38,6 +39,9 @@ namespace mjk {
 int launch_jk(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, hipStream_t st);
 int launch_jk_rows(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, size_t shmem, hipStream_t st);
 int row_kernel_available(int la, int lb, int lc, int ld);
+int launch_jk_rows_gc(int la, int lb, int lc, int ld, int ca, int cb, int cc, int cd, int nblocks, const JKArgs& a,
+                      size_t shmem, hipStream_t st);
+int row_gc_available(int la, int lb, int lc, int ld, int ca, int cb, int cc, int cd);
 int coop_kernel(int la, int lb, int lc, int ld);
 int launch_qcond(int la, int lb, const QCondArgs& a, hipStream_t st);
 int upload_c2s(hipStream_t st);
@@ -70,7 +74,22 @@ constexpr int ROW_LDS_MAX = 160 * 1024 - 2048;
 constexpr double EXPCUTOFF_MAX = 60.0;
 constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8, PTR_COORD = 1, ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2,
               NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6, PTR_RANGE_OMEGA = 8;
-constexpr int NQCLASS_MAX = NPAIRCLASS * (NPAIRCLASS + 1) / 2;
+// Quartet classes of a basis: one per pair of pair classes and pair groups
+// (at most 5 x 5 group combinations for a pair of classes with every group).
+constexpr int NQCLASS_MAX = 1024;
+
+// Pair groups within a pair class.  Shells of the input basis with one
+// contraction, and generally contracted shells that are not merged (l > 0 or
+// more than two contractions), are split into segmented shells (one per
+// contraction).  A generally contracted s shell with two contractions is a
+// "merged unit"; its pairs exist twice: split into segmented pairs (group
+// SPLIT) and as unit pairs carrying both contractions (groups M21, M12, M22:
+// the contraction counts of A and B).  Pairs of segmented shells not from a
+// merged unit form group PLAIN, shared by both representations.
+enum PairGroup { G_PLAIN = 0, G_SPLIT = 1, G_M21 = 2, G_M12 = 3, G_M22 = 4, NGT = 5 };
+constexpr int NGROUP = NPAIRCLASS * NGT;
+__host__ __device__ constexpr int group_ca(int t) { return t == G_M21 || t == G_M22 ? 2 : 1; }
+__host__ __device__ constexpr int group_cb(int t) { return t == G_M12 || t == G_M22 ? 2 : 1; }
 
 // Shell pairs of a class are grouped by their number of primitive pairs so
 // that every tile runs a uniform primitive loop (no lane divergence).
@@ -89,17 +108,32 @@ struct GShell {
     double r[3];
     std::vector<double> e, c;
     int s0, c0;   // molecule-local spherical / Cartesian offsets
+    int unit;     // merged unit (or lone segmented shell) this shell belongs to
+    int ci;       // contraction index within its shell of the input basis
+    bool merged;  // part of a merged two-contraction s shell
+};
+
+// A unit is a merged two-contraction s shell or a lone segmented shell.
+struct UnitHost {
+    int first;    // first segmented shell
+    int nctr;     // 2 for a merged unit, 1 otherwise
 };
 
 struct MolHost {
     int nao = 0, ncart = 0, nsh = 0, nbas_orig = 0;
     std::vector<GShell> sh;
+    std::vector<UnitHost> units;
+    std::vector<int> orig_s0, orig_n;   // per input shell: first spherical AO, spherical AOs (all contractions)
     long long sbase = 0, dbase = 0, cbase = 0;
     long long sph_first = 0, cart_first = 0, sh_first = 0;
 };
 
 struct QClass {
     int X, Y, la, lb, lc, ld;
+    int GX = 0, GY = 0;       // pair groups (X * NGT + type) of bra and ket
+    int ca = 1, cb = 1, cc = 1, cd = 1;   // contraction counts of the groups
+    int mode = 0;             // 0 both representations, 1 split only, 2 merged only
+    int base = 0;             // index of the (X, Y) class this one belongs to
     int nbt = 0;
     long long nblocks = 0;
     double work = 0;          // rough cost (ket tiles x components x roots)
@@ -122,13 +156,16 @@ int lmax_built()
     return lmax_kernels();
 }
 
-double flops_class(int a, int b, int c, int d, double nq, double nprimq)
+// Algorithmic FLOPs of nq evaluated quartets with nprimq primitive quartets;
+// ncomb > 1: a merged class, whose primitive quartets each feed ncomb
+// contraction combinations (nq counts every combination).
+double flops_class(int a, int b, int c, int d, double nq, double nprimq, int ncomb = 1)
 {
     const int L = a + b + c + d;
     const double nr = L / 2 + 1;
     const double Nc = (double)ncart(a) * ncart(b) * ncart(c) * ncart(d);
     const double Ns = (double)nsph(a) * nsph(b) * nsph(c) * nsph(d);
-    const double fprim = 30 + 20 * nr + 12 * nr * (a + b + 1) * (c + d + 1) + 3 * nr * Nc + 2 * Nc;
+    const double fprim = 30 + 20 * nr + 12 * nr * (a + b + 1) * (c + d + 1) + 3 * nr * Nc + 2 * ncomb * Nc;
     double fc2s = 0;
     if (a >= 2) fc2s += (double)nsph(a) * ncart(a) * ncart(b) * ncart(c) * ncart(d);
     if (b >= 2) fc2s += (double)nsph(a) * nsph(b) * ncart(b) * ncart(c) * ncart(d);
@@ -312,12 +349,15 @@ struct mjk_basis {
     int npairs = 0;
     double4* ppP = nullptr;
     double2* ppK = nullptr;
+    double4* ppW = nullptr;          // merged pairs' coefficient products (PairSet::ppW)
     int *prim0 = nullptr, *nprim = nullptr, *shA = nullptr, *shB = nullptr, *cartA = nullptr, *cartB = nullptr;
+    int *oshA = nullptr, *oshB = nullptr;
     double4* tP = nullptr;           // ket-side transposed primitive pairs (PairSet::tP)
     double2* tK = nullptr;
+    double4* tW = nullptr;
     int *tpos = nullptr, *tstride = nullptr;
     double4 *rA = nullptr, *rAB = nullptr;
-    std::vector<int> seg;            // [(class * nmol + mol) * NBUCKET + bucket] -> first pair
+    std::vector<int> seg;            // [(group * nmol + mol) * NBUCKET + bucket] -> first pair, group = class * NGT + type
     int* d_pmol = nullptr;           // molecule of every pair
     int* d_active = nullptr;         // per molecule: 0 skips its quartets (mjk_basis_set_active)
     int class_filter = -1;           // >= 0: builds run only this quartet class (mjk_basis_set_class_filter)
@@ -375,185 +415,236 @@ void free_tiles(mjk_basis* b)
 }
 
 // Bra tiles of TILE pairs; each keeps the ket prefix whose Schwarz product
-// with the tile's largest bound survives the cutoff.
+// with the tile's largest bound survives the cutoff.  One quartet class per
+// pair of pair groups: for the (X, Y) pair classes, the plain x plain class
+// always runs; where merged units exist, either the classes over the split
+// groups (mode 1) or those over the merged groups (mode 2) run, never both.
+// Within one group (bra and ket of the same group) only q <= p is kept; across
+// groups every combination, each in one class.
 int build_tiles(mjk_basis* b, double cutoff)
 {
     mjk_ctx* ctx = b->ctx;
     free_tiles(b);
     const int nmol = b->nmol;
-    for (int X = 0; X < NPAIRCLASS; ++X) {
-        for (int Y = 0; Y <= X; ++Y) {
-            QClass qc;
-            qc.X = X;
-            qc.Y = Y;
-            // decode (la, lb) of a pair class
-            auto dec = [](int c, int& la, int& lb) {
-                la = 0;
-                while ((la + 1) * (la + 2) / 2 <= c) ++la;
-                lb = c - la * (la + 1) / 2;
-            };
-            dec(X, qc.la, qc.lb);
-            dec(Y, qc.lc, qc.ld);
-            std::vector<int4> bt;
-            std::vector<int> btm, off(1, 0);
-            long long total = 0;
-            double est_q = 0, est_p = 0;   // Schwarz-prescreened quartets and primitive quartets
-            for (int m = 0; m < nmol; ++m)
-            for (int bx = 0; bx < NBUCKET; ++bx) {
-                const size_t sx = ((size_t)X * nmol + m) * NBUCKET + bx;
-                const int x0 = b->seg[sx], x1 = b->seg[sx + 1];
-                for (int p0 = x0; p0 < x1; p0 += TILE)
-                for (int by = 0; by < NBUCKET; ++by) {
-                    const size_t sy = ((size_t)Y * nmol + m) * NBUCKET + by;
-                    const int y0 = b->seg[sy], y1 = b->seg[sy + 1];
-                    if (y1 <= y0) continue;
-                    const int p1 = std::min(p0 + TILE, x1);
-                    int qend = y1;
-                    if (cutoff > 0) {
-                        const double qb = b->q[p0];
-                        // first ket index with q[y] * qb <= cutoff (q sorted descending)
-                        int lo = y0, hi = y1;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) / 2;
-                            if (b->q[mid] * qb > cutoff) lo = mid + 1; else hi = mid;
-                        }
-                        qend = lo;
+    auto has = [&](int G) {
+        return b->seg[(size_t)(G + 1) * nmol * NBUCKET] > b->seg[(size_t)G * nmol * NBUCKET];
+    };
+    // decode (la, lb) of a pair class
+    auto dec = [](int c, int& la, int& lb) {
+        la = 0;
+        while ((la + 1) * (la + 2) / 2 <= c) ++la;
+        lb = c - la * (la + 1) / 2;
+    };
+    auto make_class = [&](QClass& qc, int GX, int GY) -> int {
+        std::vector<int4> bt;
+        std::vector<int> btm, off(1, 0);
+        long long total = 0;
+        double est_q = 0, est_p = 0;   // Schwarz-prescreened quartets and primitive quartets
+        for (int m = 0; m < nmol; ++m)
+        for (int bx = 0; bx < NBUCKET; ++bx) {
+            const size_t sx = ((size_t)GX * nmol + m) * NBUCKET + bx;
+            const int x0 = b->seg[sx], x1 = b->seg[sx + 1];
+            for (int p0 = x0; p0 < x1; p0 += TILE)
+            for (int by = 0; by < NBUCKET; ++by) {
+                const size_t sy = ((size_t)GY * nmol + m) * NBUCKET + by;
+                const int y0 = b->seg[sy], y1 = b->seg[sy + 1];
+                if (y1 <= y0) continue;
+                const int p1 = std::min(p0 + TILE, x1);
+                int qend = y1;
+                if (cutoff > 0) {
+                    const double qb = b->q[p0];
+                    // first ket index with q[y] * qb <= cutoff (q sorted descending)
+                    int lo = y0, hi = y1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (b->q[mid] * qb > cutoff) lo = mid + 1; else hi = mid;
                     }
-                    if (X == Y) qend = std::min(qend, p1);
-                    const int nkt = (qend - y0 + TILE - 1) / TILE;
-                    if (nkt <= 0) continue;
-                    bt.push_back(make_int4(p0, p1, y0, qend));
-                    btm.push_back(m);
-                    for (int p = p0; p < p1; ++p) {
-                        const int qe = X == Y ? std::min(qend, p + 1) : qend;
-                        if (qe > y0) {
-                            est_q += qe - y0;
-                            est_p += (double)b->h_nprim[p] * (b->pcost[qe] - b->pcost[y0]);
-                        }
-                    }
-                    total += nkt;
-                    if (total > 2000000000LL)
-                        return fail(ctx, MJK_EUNSUPPORTED, "quartet class has more than 2^31 tiles");
-                    off.push_back((int)total);
+                    qend = lo;
                 }
-            }
-            qc.nbt = (int)bt.size();
-            qc.nblocks = total;
-            qc.work = (double)total * ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) * ncart(qc.ld) *
-                      ((qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1);
-            qc.est = flops_class(qc.la, qc.lb, qc.lc, qc.ld, est_q, est_p);
-            if (qc.nbt > 0) {
-                int rc;
-                if ((rc = upload(ctx, bt, &qc.bt))) return rc;
-                if ((rc = upload(ctx, btm, &qc.bt_mol))) return rc;
-                if ((rc = upload(ctx, off, &qc.bt_off))) return rc;
-            }
-            // Row-kernel tasks: the same Schwarz prefixes per bra pair, cut into
-            // chunks of about equal primitive-quartet count (the kets' primitive
-            // counts differ up to a hundredfold).
-            if (qc.nbt > 0 && row_kernel_available(qc.la, qc.lb, qc.lc, qc.ld)) {
-                std::vector<int2> rr;
-                std::vector<int> bra;
-                long long cost_all = 0;
-                for (int m = 0; m < nmol; ++m) {
-                    const size_t sx = ((size_t)X * nmol + m) * NBUCKET, sy = ((size_t)Y * nmol + m) * NBUCKET;
-                    for (int p = b->seg[sx]; p < b->seg[sx + NBUCKET]; ++p) {
-                        long long np_p = 0;
-                        for (int by = 0; by < NBUCKET; ++by) {
-                            const int y0 = b->seg[sy + by], y1 = b->seg[sy + by + 1];
-                            int yend = y1;
-                            if (cutoff > 0) {
-                                int lo = y0, hi = y1;
-                                while (lo < hi) {
-                                    const int mid = (lo + hi) / 2;
-                                    if (b->q[mid] * b->q[p] > cutoff) lo = mid + 1; else hi = mid;
-                                }
-                                yend = lo;
-                            }
-                            if (X == Y) yend = std::min(yend, p + 1);
-                            yend = std::max(yend, y0);
-                            rr.push_back(make_int2(y0, yend));
-                            np_p += b->pcost[yend] - b->pcost[y0];
-                        }
-                        if (np_p == 0) {
-                            rr.resize(rr.size() - NBUCKET);
-                            continue;
-                        }
-                        bra.push_back(p);
-                        cost_all += (long long)b->h_nprim[p] * np_p;
+                if (GX == GY) qend = std::min(qend, p1);
+                const int nkt = (qend - y0 + TILE - 1) / TILE;
+                if (nkt <= 0) continue;
+                bt.push_back(make_int4(p0, p1, y0, qend));
+                btm.push_back(m);
+                for (int p = p0; p < p1; ++p) {
+                    const int qe = GX == GY ? std::min(qend, p + 1) : qend;
+                    if (qe > y0) {
+                        est_q += qe - y0;
+                        est_p += (double)b->h_nprim[p] * (b->pcost[qe] - b->pcost[y0]);
                     }
                 }
-                // enough blocks to fill the chip, each long enough to amortise its
-                // fixed cost (LDS clear, range set-up, row flush)
-                const long long budget = std::min(std::max(cost_all / ctx->row_blocks, (long long)ctx->row_chunk),
-                                                  std::max(cost_all / ctx->row_min_blocks, 1LL));
-                std::vector<std::pair<long long, int4>> tasks;
-                std::vector<int> tprim;   // bra primitive offset of every task (parallel to tasks)
-                // bra primitive pairs per task: small classes give every bra primitive
-                // pair its own task (a small molecule's long contractions are its
-                // critical path: H2O cc-pVDZ 0.69 -> 0.52 ms per build from 2 to 1);
-                // large ones share them in eights (measured on MI355X, (H2O)16 def2-SVP
-                // with the classes on four streams: 2 -> 92, 4 -> 107, 8 -> 110 builds/s)
-                const int P = ctx->row_prim > 0 ? ctx->row_prim
-                              : cost_all < (1LL << 18) ? 1 : cost_all < (1LL << 20) ? 2 : 8;
-                qc.rprim = P;
-                for (size_t ib = 0; ib < bra.size(); ++ib)
-                for (int pbo = 0; pbo < b->h_nprim[bra[ib]]; pbo += P) {
-                    // bra pair ib, its primitive pairs [pbo, pbo + P)
-                    const int p = bra[ib];
-                    const long long npb = std::min(P, b->h_nprim[p] - pbo);
-                    int flat = 0, start = 0;
-                    long long cur = 0;
-                    for (int k = 0; k < NBUCKET; ++k) {
-                        int y = rr[ib * NBUCKET + k].x;
-                        const int y1 = rr[ib * NBUCKET + k].y;
-                        while (y < y1) {
-                            const long long rest = npb * (b->pcost[y1] - b->pcost[y]);
-                            if (cur + rest < budget) {
-                                cur += rest;
-                                flat += y1 - y;
-                                break;
-                            }
-                            // smallest y' with npb * (pcost[y'] - pcost[y]) >= budget - cur
-                            int lo = y + 1, hi = y1;
+                total += nkt;
+                if (total > 2000000000LL)
+                    return fail(ctx, MJK_EUNSUPPORTED, "quartet class has more than 2^31 tiles");
+                off.push_back((int)total);
+            }
+        }
+        qc.nbt = (int)bt.size();
+        qc.nblocks = total;
+        qc.work = (double)total * ncart(qc.la) * ncart(qc.lb) * ncart(qc.lc) * ncart(qc.ld) *
+                  ((qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1);
+        qc.est = flops_class(qc.la, qc.lb, qc.lc, qc.ld, est_q * qc.ca * qc.cb * qc.cc * qc.cd, est_p,
+                             qc.ca * qc.cb * qc.cc * qc.cd);
+        if (qc.nbt > 0) {
+            int rc;
+            if ((rc = upload(ctx, bt, &qc.bt))) return rc;
+            if ((rc = upload(ctx, btm, &qc.bt_mol))) return rc;
+            if ((rc = upload(ctx, off, &qc.bt_off))) return rc;
+        }
+        // Row-kernel tasks: the same Schwarz prefixes per bra pair, cut into
+        // chunks of about equal primitive-quartet count (the kets' primitive
+        // counts differ up to a hundredfold).
+        if (qc.nbt > 0 && (qc.ca * qc.cb * qc.cc * qc.cd > 1
+                              ? row_gc_available(qc.la, qc.lb, qc.lc, qc.ld, qc.ca, qc.cb, qc.cc, qc.cd)
+                              : row_kernel_available(qc.la, qc.lb, qc.lc, qc.ld))) {
+            std::vector<int2> rr;
+            std::vector<int> bra;
+            long long cost_all = 0;
+            for (int m = 0; m < nmol; ++m) {
+                const size_t sx = ((size_t)GX * nmol + m) * NBUCKET, sy = ((size_t)GY * nmol + m) * NBUCKET;
+                for (int p = b->seg[sx]; p < b->seg[sx + NBUCKET]; ++p) {
+                    long long np_p = 0;
+                    for (int by = 0; by < NBUCKET; ++by) {
+                        const int y0 = b->seg[sy + by], y1 = b->seg[sy + by + 1];
+                        int yend = y1;
+                        if (cutoff > 0) {
+                            int lo = y0, hi = y1;
                             while (lo < hi) {
                                 const int mid = (lo + hi) / 2;
-                                if (npb * (b->pcost[mid] - b->pcost[y]) >= budget - cur) hi = mid; else lo = mid + 1;
+                                if (b->q[mid] * b->q[p] > cutoff) lo = mid + 1; else hi = mid;
                             }
-                            cur += npb * (b->pcost[lo] - b->pcost[y]);
-                            flat += lo - y;
-                            y = lo;
-                            tasks.push_back({cur, make_int4(p, start, flat, (int)ib)});
-                            tprim.push_back(pbo);
-                            start = flat;
-                            cur = 0;
+                            yend = lo;
                         }
+                        if (GX == GY) yend = std::min(yend, p + 1);
+                        yend = std::max(yend, y0);
+                        rr.push_back(make_int2(y0, yend));
+                        np_p += b->pcost[yend] - b->pcost[y0];
                     }
-                    if (flat > start) {
+                    if (np_p == 0) {
+                        rr.resize(rr.size() - NBUCKET);
+                        continue;
+                    }
+                    bra.push_back(p);
+                    cost_all += (long long)b->h_nprim[p] * np_p;
+                }
+            }
+            // enough blocks to fill the chip, each long enough to amortise its
+            // fixed cost (LDS clear, range set-up, row flush)
+            const long long budget = std::min(std::max(cost_all / ctx->row_blocks, (long long)ctx->row_chunk),
+                                              std::max(cost_all / ctx->row_min_blocks, 1LL));
+            std::vector<std::pair<long long, int4>> tasks;
+            std::vector<int> tprim;   // bra primitive offset of every task (parallel to tasks)
+            // bra primitive pairs per task: small classes give every bra primitive
+            // pair its own task (a small molecule's long contractions are its
+            // critical path: H2O cc-pVDZ 0.69 -> 0.52 ms per build from 2 to 1);
+            // large ones share them in eights (measured on MI355X, (H2O)16 def2-SVP
+            // with the classes on four streams: 2 -> 92, 4 -> 107, 8 -> 110 builds/s)
+            const int P = ctx->row_prim > 0 ? ctx->row_prim
+                          : cost_all < (1LL << 18) ? 1 : cost_all < (1LL << 20) ? 2 : 8;
+            qc.rprim = P;
+            for (size_t ib = 0; ib < bra.size(); ++ib)
+            for (int pbo = 0; pbo < b->h_nprim[bra[ib]]; pbo += P) {
+                // bra pair ib, its primitive pairs [pbo, pbo + P)
+                const int p = bra[ib];
+                const long long npb = std::min(P, b->h_nprim[p] - pbo);
+                int flat = 0, start = 0;
+                long long cur = 0;
+                for (int k = 0; k < NBUCKET; ++k) {
+                    int y = rr[ib * NBUCKET + k].x;
+                    const int y1 = rr[ib * NBUCKET + k].y;
+                    while (y < y1) {
+                        const long long rest = npb * (b->pcost[y1] - b->pcost[y]);
+                        if (cur + rest < budget) {
+                            cur += rest;
+                            flat += y1 - y;
+                            break;
+                        }
+                        // smallest y' with npb * (pcost[y'] - pcost[y]) >= budget - cur
+                        int lo = y + 1, hi = y1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) / 2;
+                            if (npb * (b->pcost[mid] - b->pcost[y]) >= budget - cur) hi = mid; else lo = mid + 1;
+                        }
+                        cur += npb * (b->pcost[lo] - b->pcost[y]);
+                        flat += lo - y;
+                        y = lo;
                         tasks.push_back({cur, make_int4(p, start, flat, (int)ib)});
                         tprim.push_back(pbo);
+                        start = flat;
+                        cur = 0;
                     }
-                    tasks.back().second.z = INT_MAX;   // the last chunk runs to the end of the kets
                 }
-                // heaviest tasks first (the last chunk of a bra pair is the short one)
-                std::vector<size_t> order(tasks.size());
-                std::iota(order.begin(), order.end(), 0);
-                std::stable_sort(order.begin(), order.end(),
-                                 [&](size_t u, size_t v) { return tasks[u].first > tasks[v].first; });
-                if (tasks.size() > 2000000000ULL) return fail(ctx, MJK_EUNSUPPORTED, "too many row-kernel tasks");
-                std::vector<int4> rt(tasks.size());
-                std::vector<int> rtp(tasks.size());
-                for (size_t t = 0; t < tasks.size(); ++t) {
-                    rt[t] = tasks[order[t]].second;
-                    rtp[t] = tprim[order[t]];
+                if (flat > start) {
+                    tasks.push_back({cur, make_int4(p, start, flat, (int)ib)});
+                    tprim.push_back(pbo);
                 }
-                qc.nrt = (int)rt.size();
-                int rc;
-                if (qc.nrt > 0 && ((rc = upload(ctx, rt, &qc.rt)) || (rc = upload(ctx, rr, &qc.rr)) ||
-                                   (rc = upload(ctx, rtp, &qc.rtp))))
-                    return rc;
+                tasks.back().second.z = INT_MAX;   // the last chunk runs to the end of the kets
             }
-            b->qcls.push_back(qc);
+            // heaviest tasks first (the last chunk of a bra pair is the short one)
+            std::vector<size_t> order(tasks.size());
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](size_t u, size_t v) { return tasks[u].first > tasks[v].first; });
+            if (tasks.size() > 2000000000ULL) return fail(ctx, MJK_EUNSUPPORTED, "too many row-kernel tasks");
+            std::vector<int4> rt(tasks.size());
+            std::vector<int> rtp(tasks.size());
+            for (size_t t = 0; t < tasks.size(); ++t) {
+                rt[t] = tasks[order[t]].second;
+                rtp[t] = tprim[order[t]];
+            }
+            qc.nrt = (int)rt.size();
+            int rc;
+            if (qc.nrt > 0 && ((rc = upload(ctx, rt, &qc.rt)) || (rc = upload(ctx, rr, &qc.rr)) ||
+                               (rc = upload(ctx, rtp, &qc.rtp))))
+                return rc;
+        }
+        return MJK_OK;
+    };
+    int nbase = 0;
+    for (int X = 0; X < NPAIRCLASS; ++X) {
+        for (int Y = 0; Y <= X; ++Y, ++nbase) {
+            int la, lb, lc, ld;
+            dec(X, la, lb);
+            dec(Y, lc, ld);
+            const bool gc = has(X * NGT + G_SPLIT) || has(Y * NGT + G_SPLIT);
+            std::vector<std::pair<int, int>> split, merged;   // (bra type, ket type)
+            for (int tx = 0; tx < NGT; ++tx)
+                for (int ty = 0; ty < NGT; ++ty) {
+                    if (X == Y && ty > tx) continue;
+                    if (!has(X * NGT + tx) || !has(Y * NGT + ty)) continue;
+                    if (tx == G_PLAIN && ty == G_PLAIN) continue;
+                    const bool sx = tx <= G_SPLIT, sy = ty <= G_SPLIT;
+                    if (sx && sy) split.push_back({tx, ty});
+                    else if ((tx == G_PLAIN || !sx) && (ty == G_PLAIN || !sy)) merged.push_back({tx, ty});
+                }
+            // merged mode needs a merged row kernel for every one of its classes
+            // (the Coulomb operator only)
+            bool merged_ok = gc && b->omega == 0.0 && !merged.empty();
+            for (auto& t : merged)
+                merged_ok = merged_ok && row_gc_available(la, lb, lc, ld, group_ca(t.first), group_cb(t.first),
+                                                          group_ca(t.second), group_cb(t.second));
+            std::vector<std::tuple<int, int, int>> subs;     // (bra type, ket type, mode)
+            subs.push_back(std::make_tuple((int)G_PLAIN, (int)G_PLAIN, 0));
+            for (auto& t : split) subs.push_back(std::make_tuple(t.first, t.second, merged_ok ? 1 : 0));
+            if (merged_ok)
+                for (auto& t : merged) subs.push_back(std::make_tuple(t.first, t.second, 2));
+            for (auto& sub : subs) {
+                QClass qc;
+                qc.X = X;
+                qc.Y = Y;
+                qc.la = la, qc.lb = lb, qc.lc = lc, qc.ld = ld;
+                const int tx = std::get<0>(sub), ty = std::get<1>(sub);
+                qc.GX = X * NGT + tx;
+                qc.GY = Y * NGT + ty;
+                qc.ca = group_ca(tx), qc.cb = group_cb(tx), qc.cc = group_ca(ty), qc.cd = group_cb(ty);
+                qc.mode = std::get<2>(sub);
+                qc.base = nbase;
+                int rc;
+                if ((rc = make_class(qc, qc.GX, qc.GY))) return rc;
+                b->qcls.push_back(qc);
+                if ((int)b->qcls.size() > NQCLASS_MAX)
+                    return fail(ctx, MJK_EUNSUPPORTED, "more than %d quartet classes", NQCLASS_MAX);
+            }
         }
     }
     b->tile_cutoff = cutoff;
@@ -565,16 +656,20 @@ PairSet pairset(const mjk_basis* b)
     PairSet P;
     P.ppP = b->ppP;
     P.ppK = b->ppK;
+    P.ppW = b->ppW;
     P.prim0 = b->prim0;
     P.nprim = b->nprim;
     P.rA = b->rA;
     P.rAB = b->rAB;
     P.shA = b->shA;
     P.shB = b->shB;
+    P.oshA = b->oshA;
+    P.oshB = b->oshB;
     P.cartA = b->cartA;
     P.cartB = b->cartB;
     P.tP = b->tP;
     P.tK = b->tK;
+    P.tW = b->tW;
     P.tpos = b->tpos;
     P.tstride = b->tstride;
     return P;
@@ -627,6 +722,14 @@ int track_upload(mjk_basis* b, const std::vector<T>& v, T** d)
     return rc;
 }
 
+// Dynamic LDS of a row-kernel block of class q for n_j J and n_k K densities:
+// the bra's K' rows and its J'[a,b] block (every contraction of a merged unit).
+size_t row_shmem(const QClass& q, int n_j, int n_k, int maxld)
+{
+    const size_t ra = (size_t)q.ca * ncart(q.la), rb = (size_t)q.cb * ncart(q.lb);
+    return sizeof(double) * ((size_t)n_k * (ra + rb) * maxld + (size_t)n_j * ra * rb);
+}
+
 int parse_molecule(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas, const double* env,
                    MolHost& m)
 {
@@ -640,11 +743,20 @@ int parse_molecule(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int n
         if (np <= 0 || nc <= 0 || ia < 0 || ia >= natm) return fail(ctx, MJK_EINVAL, "bad shell %d", i);
         const double* xyz = env + atm[ia * ATM_SLOTS + PTR_COORD];
         const double ang = (l == 0) ? C2S_S0 : (l == 1) ? C2S_P1 : 1.0;
+        // a two-contraction s shell is kept whole as a merged unit
+        const bool merged = l == 0 && nc == 2;
+        m.orig_s0.push_back(s0);
+        m.orig_n.push_back(nc * nsph(l));
+        if (merged) m.units.push_back({(int)m.sh.size(), 2});
         for (int c = 0; c < nc; ++c) {
             GShell g;
             g.l = l;
             g.nprim = np;
             g.orig = i;
+            g.ci = c;
+            g.merged = merged;
+            if (!merged) m.units.push_back({(int)m.sh.size(), 1});
+            g.unit = (int)m.units.size() - 1;
             g.r[0] = xyz[0];
             g.r[1] = xyz[1];
             g.r[2] = xyz[2];
@@ -665,10 +777,14 @@ int parse_molecule(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int n
 }
 
 struct PairHost {
-    int A, B;          // molecule-local shell indices, l(A) >= l(B)
+    int A, B;          // molecule-local (first) shell indices, l(A) >= l(B)
     int mol;
+    int type = G_PLAIN;   // PairGroup
+    int oA = 0, oB = 0;   // shells of the input basis (dm_cond)
+    int uA = 0, uB = 0;   // units
     std::vector<double4> P;
     std::vector<double2> K;
+    std::vector<double4> W;   // merged pairs: coefficient products of the combinations
 };
 
 int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
@@ -685,13 +801,13 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
         m.sh_first = hf;
         sb += (long long)m.nao * m.nao;
         cb += (long long)m.ncart * m.ncart;
-        kb += (long long)m.nsh * m.nsh;
+        kb += (long long)m.nbas_orig * m.nbas_orig;   // dm_cond over the input basis's shells
         sf += m.nao;
         cf += m.ncart;
-        hf += m.nsh;
+        hf += m.nbas_orig;
         b->maxld = std::max(b->maxld, m.ncart);
         b->maxnao = std::max(b->maxnao, m.nao);
-        b->maxnsh = std::max(b->maxnsh, m.nsh);
+        b->maxnsh = std::max(b->maxnsh, m.nbas_orig);
     }
     b->slot_s = sb;
     b->slot_c = cb;
@@ -700,8 +816,8 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     b->ncart_tot = cf;
     b->nsh_tot = hf;
 
-    // shell pairs, grouped by class then molecule
-    std::vector<std::vector<PairHost>> byseg((size_t)NPAIRCLASS * nmol * NBUCKET);
+    // shell pairs, grouped by class and pair group, then molecule
+    std::vector<std::vector<PairHost>> byseg((size_t)NGROUP * nmol * NBUCKET);
     const double sq2pi54 = std::sqrt(2.0) * std::pow(M_PI, 1.25);
     for (int mi = 0; mi < nmol; ++mi) {
         const MolHost& m = b->mols[mi];
@@ -717,6 +833,11 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
                 ph.A = A;
                 ph.B = B;
                 ph.mol = mi;
+                ph.type = sa.merged || sbb.merged ? G_SPLIT : G_PLAIN;
+                ph.oA = sa.orig;
+                ph.oB = sbb.orig;
+                ph.uA = sa.unit;
+                ph.uB = sbb.unit;
                 for (int ia = 0; ia < sa.nprim; ++ia)
                     for (int ib = 0; ib < sbb.nprim; ++ib) {
                         const double ea = sa.e[ia], eb = sbb.e[ib];
@@ -735,13 +856,67 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
                     }
                 if (ph.P.empty()) continue;
                 const int bk = prim_bucket((int)ph.P.size());
-                byseg[((size_t)pair_class(sa.l, sbb.l) * nmol + mi) * NBUCKET + bk].push_back(std::move(ph));
+                const int G = pair_class(sa.l, sbb.l) * NGT + ph.type;
+                byseg[((size_t)G * nmol + mi) * NBUCKET + bk].push_back(std::move(ph));
+            }
+        }
+        // merged pairs: pairs of units at least one of which is a merged s shell;
+        // l(A) >= l(B), and for two s units the merged one (or the later one) first
+        const int nu = (int)m.units.size();
+        for (int u = 0; u < nu; ++u) {
+            for (int v = 0; v <= u; ++v) {
+                if (m.units[u].nctr == 1 && m.units[v].nctr == 1) continue;
+                int uA = u, uB = v;
+                const int lu = m.sh[m.units[u].first].l, lv = m.sh[m.units[v].first].l;
+                if (lu < lv || (lu == lv && m.units[u].nctr < m.units[v].nctr)) std::swap(uA, uB);
+                const UnitHost& UA = m.units[uA];
+                const UnitHost& UB = m.units[uB];
+                const GShell& sa = m.sh[UA.first];
+                const GShell& sbb = m.sh[UB.first];
+                const double ABx = sa.r[0] - sbb.r[0], ABy = sa.r[1] - sbb.r[1], ABz = sa.r[2] - sbb.r[2];
+                const double rr = ABx * ABx + ABy * ABy + ABz * ABz;
+                PairHost ph;
+                ph.A = UA.first;
+                ph.B = UB.first;
+                ph.mol = mi;
+                ph.type = UA.nctr == 2 && UB.nctr == 2 ? G_M22 : UA.nctr == 2 ? G_M21 : G_M12;
+                ph.oA = sa.orig;
+                ph.oB = sbb.orig;
+                ph.uA = uA;
+                ph.uB = uB;
+                for (int ia = 0; ia < sa.nprim; ++ia)
+                    for (int ib = 0; ib < sbb.nprim; ++ib) {
+                        const double ea = sa.e[ia], eb = sbb.e[ib];
+                        const double p = ea + eb;
+                        const double E = ea * eb / p * rr;
+                        if (E > ctx->expcutoff) continue;
+                        double w[4] = {0, 0, 0, 0};
+                        bool any = false;
+                        for (int ci = 0; ci < UA.nctr; ++ci)
+                            for (int cj = 0; cj < UB.nctr; ++cj) {
+                                w[ci * UB.nctr + cj] = m.sh[UA.first + ci].c[ia] * m.sh[UB.first + cj].c[ib];
+                                any = any || w[ci * UB.nctr + cj] != 0.0;
+                            }
+                        if (!any) continue;
+                        double4 P;
+                        P.x = (ea * sa.r[0] + eb * sbb.r[0]) / p;
+                        P.y = (ea * sa.r[1] + eb * sbb.r[1]) / p;
+                        P.z = (ea * sa.r[2] + eb * sbb.r[2]) / p;
+                        P.w = p;
+                        ph.P.push_back(P);
+                        ph.K.push_back(make_double2(std::exp(-E) * sq2pi54 / p, 1.0 / p));
+                        ph.W.push_back(make_double4(w[0], w[1], w[2], w[3]));
+                    }
+                if (ph.P.empty()) continue;
+                const int bk = prim_bucket((int)ph.P.size());
+                const int G = pair_class(sa.l, sbb.l) * NGT + ph.type;
+                byseg[((size_t)G * nmol + mi) * NBUCKET + bk].push_back(std::move(ph));
             }
         }
     }
     // flatten (unsorted) for the Schwarz kernel
     auto flatten = [&](std::vector<int>& order_seg_start) {
-        order_seg_start.assign((size_t)NPAIRCLASS * nmol * NBUCKET + 1, 0);
+        order_seg_start.assign((size_t)NGROUP * nmol * NBUCKET + 1, 0);
         int n = 0;
         for (size_t s = 0; s < byseg.size(); ++s) {
             order_seg_start[s] = n;
@@ -751,9 +926,9 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
         return n;
     };
     auto upload_pairs = [&](std::vector<double>* qv) -> int {
-        std::vector<double4> hP, hA, hAB;
+        std::vector<double4> hP, hA, hAB, hW;
         std::vector<double2> hK;
-        std::vector<int> hp0, hnp, hsa, hsb, hca, hcb;
+        std::vector<int> hp0, hnp, hsa, hsb, hca, hcb, hoa, hob;
         for (auto& segv : byseg)
             for (auto& ph : segv) {
                 const MolHost& m = b->mols[ph.mol];
@@ -763,6 +938,10 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
                 hnp.push_back((int)ph.P.size());
                 hP.insert(hP.end(), ph.P.begin(), ph.P.end());
                 hK.insert(hK.end(), ph.K.begin(), ph.K.end());
+                if (ph.W.empty()) hW.resize(hW.size() + ph.P.size(), make_double4(0, 0, 0, 0));
+                else hW.insert(hW.end(), ph.W.begin(), ph.W.end());
+                hoa.push_back(ph.oA);
+                hob.push_back(ph.oB);
                 const double qq = qv ? (*qv)[hsa.size()] : 0.0;
                 hA.push_back(make_double4(sa.r[0], sa.r[1], sa.r[2], qq));
                 hAB.push_back(make_double4(sa.r[0] - sbb.r[0], sa.r[1] - sbb.r[1], sa.r[2] - sbb.r[2], 0.0));
@@ -775,6 +954,9 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
         int rc;
         if ((rc = track_upload(b, hP, &b->ppP))) return rc;
         if ((rc = track_upload(b, hK, &b->ppK))) return rc;
+        if ((rc = track_upload(b, hW, &b->ppW))) return rc;
+        if ((rc = track_upload(b, hoa, &b->oshA))) return rc;
+        if ((rc = track_upload(b, hob, &b->oshB))) return rc;
         if ((rc = track_upload(b, hp0, &b->prim0))) return rc;
         if ((rc = track_upload(b, hnp, &b->nprim))) return rc;
         if ((rc = track_upload(b, hA, &b->rA))) return rc;
@@ -786,7 +968,7 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
         if (qv) {
             // ket-side transposed copy: per segment, primitive ik of its pair j at
             // base + ik * n + j (n pairs, padded to the segment's largest count)
-            std::vector<double4> tP;
+            std::vector<double4> tP, tW;
             std::vector<double2> tK;
             std::vector<int> tpos, tstr;
             tpos.reserve(hp0.size());
@@ -798,11 +980,13 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
                 const size_t base = tP.size();
                 tP.resize(base + mx * n, make_double4(0, 0, 0, 1));
                 tK.resize(base + mx * n, make_double2(0, 1));
+                tW.resize(base + mx * n, make_double4(0, 0, 0, 0));
                 for (int j = 0; j < n; ++j) {
                     const PairHost& ph = segv[j];
                     for (size_t ik = 0; ik < ph.P.size(); ++ik) {
                         tP[base + ik * n + j] = ph.P[ik];
                         tK[base + ik * n + j] = ph.K[ik];
+                        if (!ph.W.empty()) tW[base + ik * n + j] = ph.W[ik];
                     }
                     tpos.push_back((int)(base + j));
                     tstr.push_back(n);
@@ -811,6 +995,7 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
             if (tP.size() > (size_t)2000000000) return fail(ctx, MJK_EUNSUPPORTED, "too many primitive pairs");
             if ((rc = track_upload(b, tP, &b->tP))) return rc;
             if ((rc = track_upload(b, tK, &b->tK))) return rc;
+            if ((rc = track_upload(b, tW, &b->tW))) return rc;
             if ((rc = track_upload(b, tpos, &b->tpos))) return rc;
             if ((rc = track_upload(b, tstr, &b->tstride))) return rc;
         }
@@ -820,7 +1005,8 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
         return MJK_OK;
     };
     auto release_pairs = [&]() {
-        void* ps[] = {b->ppP, b->ppK, b->prim0, b->nprim, b->rA, b->rAB, b->shA, b->shB, b->cartA, b->cartB};
+        void* ps[] = {b->ppP, b->ppK, b->ppW, b->prim0, b->nprim, b->rA, b->rAB, b->shA, b->shB, b->oshA, b->oshB,
+                      b->cartA, b->cartB};
         for (void* p : ps) {
             dfree(ctx, p);
             b->owned.erase(std::remove(b->owned.begin(), b->owned.end(), p), b->owned.end());
@@ -836,7 +1022,8 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     double* dq = nullptr;
     if ((rc = dalloc(ctx, sizeof(double) * std::max(np, 1), (void**)&dq))) return rc;
     for (int X = 0; X < NPAIRCLASS; ++X) {
-        const int p0 = seg0[(size_t)X * nmol * NBUCKET], p1 = seg0[(size_t)(X + 1) * nmol * NBUCKET];
+        // every group of the class (the bounds of merged pairs are replaced below)
+        const int p0 = seg0[(size_t)X * NGT * nmol * NBUCKET], p1 = seg0[(size_t)(X + 1) * NGT * nmol * NBUCKET];
         if (p1 <= p0) continue;
         int la = 0;
         while ((la + 1) * (la + 2) / 2 <= X) ++la;
@@ -863,6 +1050,31 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     dfree(ctx, dq);
     release_pairs();
+    // the bound of a merged pair: the largest bound of its split pairs (the
+    // reference's q_cond over the whole shell, pyscf/lib/vhf/optimizer.c:408-454)
+    {
+        std::map<std::tuple<int, int, int>, double> qmax;
+        int k = 0;
+        for (auto& segv : byseg)
+            for (auto& ph : segv) {
+                if (ph.type == G_SPLIT) {
+                    const auto key = std::make_tuple(ph.mol, std::max(ph.uA, ph.uB), std::min(ph.uA, ph.uB));
+                    auto it = qmax.find(key);
+                    if (it == qmax.end()) qmax[key] = qflat[k];
+                    else it->second = std::max(it->second, qflat[k]);
+                }
+                ++k;
+            }
+        k = 0;
+        for (auto& segv : byseg)
+            for (auto& ph : segv) {
+                if (ph.type >= G_M21) {
+                    const auto it = qmax.find(std::make_tuple(ph.mol, std::max(ph.uA, ph.uB), std::min(ph.uA, ph.uB)));
+                    qflat[k] = it == qmax.end() ? 1e-100 : it->second;
+                }
+                ++k;
+            }
+    }
     // sort each (class, molecule) segment by descending bound
     {
         int k = 0;
@@ -886,9 +1098,9 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     for (int p = 0; p < np; ++p) b->pcost[p + 1] = b->pcost[p] + b->h_nprim[p];
     {
         std::vector<int> pmol(std::max(np, 1), 0);
-        for (int X = 0; X < NPAIRCLASS; ++X)
+        for (int G = 0; G < NGROUP; ++G)
             for (int m = 0; m < nmol; ++m)
-                for (int p = b->seg[((size_t)X * nmol + m) * NBUCKET]; p < b->seg[((size_t)X * nmol + m + 1) * NBUCKET]; ++p)
+                for (int p = b->seg[((size_t)G * nmol + m) * NBUCKET]; p < b->seg[((size_t)G * nmol + m + 1) * NBUCKET]; ++p)
                     pmol[p] = m;
         if ((rc = track_upload(b, pmol, &b->d_pmol))) return rc;
         std::vector<int> act(nmol, 1);
@@ -907,7 +1119,7 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
         sfirst.push_back(m.sph_first);
         hfirst.push_back(m.sh_first);
         ld.push_back(m.ncart);
-        nsh.push_back(m.nsh);
+        nsh.push_back(m.nbas_orig);
         nao.push_back(m.nao);
         for (const GShell& g : m.sh) {
             for (int c = 0; c < ncart(g.l); ++c) {
@@ -922,9 +1134,12 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
                 sl.push_back(g.l);
                 sm.push_back(s);
             }
+        }
+        // dm_cond shells: the input basis's shells, all contractions
+        for (int I = 0; I < m.nbas_orig; ++I) {
             hmol.push_back(mi);
-            hs0.push_back(g.s0);
-            hn.push_back(nsph(g.l));
+            hs0.push_back(m.orig_s0[I]);
+            hn.push_back(m.orig_n[I]);
         }
     }
     if ((rc = track_upload(b, dbase, &b->d_dbase))) return rc;
@@ -950,6 +1165,26 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     if ((rc = dalloc(ctx, sizeof(unsigned long long) * 4 * NQCLASS_MAX, (void**)&b->counters))) return rc;
     b->owned.push_back(b->counters);
     return build_tiles(b, cutoff);
+}
+
+// Classes a build of n_j J and n_k K densities runs: the plain classes, and
+// per (X, Y) pair of pair classes either its merged classes -- when every one
+// of them has row tasks whose LDS fits -- or its split classes.
+std::vector<char> classes_to_run(const mjk_ctx* ctx, const mjk_basis* b, int n_j, int n_k)
+{
+    const int nb = b->qcls.empty() ? 0 : b->qcls.back().base + 1;
+    std::vector<char> merged(nb, 0);
+    for (const QClass& q : b->qcls)
+        if (q.mode == 2) merged[q.base] = 1;
+    for (const QClass& q : b->qcls)
+        if (q.mode == 2 && q.nblocks > 0 && (q.nrt == 0 || row_shmem(q, n_j, n_k, b->maxld) > ctx->row_lds))
+            merged[q.base] = 0;
+    std::vector<char> run(b->qcls.size(), 0);
+    for (size_t c = 0; c < b->qcls.size(); ++c) {
+        const QClass& q = b->qcls[c];
+        run[c] = q.nblocks > 0 && (q.mode == 0 || (q.mode == 1) != (merged[q.base] != 0));
+    }
+    return run;
 }
 
 int ensure(mjk_ctx* ctx, double** p, long long& cap, long long need)
@@ -1152,9 +1387,9 @@ int mjk_basis_q_cond(mjk_ctx* ctx, const mjk_basis* b, int mol, double* q_cond)
     const MolHost& m = b->mols[mol];
     const int n = m.nbas_orig;
     for (int i = 0; i < n * n; ++i) q_cond[i] = 1e-100;
-    for (int X = 0; X < NPAIRCLASS; ++X) {
-        const int p0 = b->seg[((size_t)X * b->nmol + mol) * NBUCKET];
-        const int p1 = b->seg[((size_t)X * b->nmol + mol + 1) * NBUCKET];
+    for (int G = 0; G < NGROUP; ++G) {
+        const int p0 = b->seg[((size_t)G * b->nmol + mol) * NBUCKET];
+        const int p1 = b->seg[((size_t)G * b->nmol + mol + 1) * NBUCKET];
         for (int p = p0; p < p1; ++p) {
             const int i = m.sh[b->h_shA[p]].orig, j = m.sh[b->h_shB[p]].orig;
             q_cond[i * n + j] = std::max(q_cond[i * n + j], b->q[p]);
@@ -1194,6 +1429,9 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     const bool need_cond = screen || fp32_tol > 0;
     if (need_cond && (rc = ensure(ctx, &b->dmcond, b->cap_cond, b->cond_total))) return rc;
     hipStream_t st = ctx->stream;
+    std::vector<char> run = classes_to_run(ctx, b, n_j, n_k);
+    if (b->class_filter >= 0)
+        for (size_t c = 0; c < run.size(); ++c) run[c] = run[c] && (int)c == b->class_filter;
     const bool prof = b->profiling && b->pev.size() >= 2 * b->qcls.size();
     const bool tally = stats || (prof && b->prof_counts);
     // The whole build on the context's stream and the streams it forks.
@@ -1254,8 +1492,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     {
         double load[1 + mjk_ctx::NAUX] = {0, 0, 0, 0};
         for (size_t c : corder) {
-            if (b->qcls[c].nblocks == 0) continue;
-            if (b->class_filter >= 0 && (int)c != b->class_filter) continue;
+            if (!run[c]) continue;
             const int si = (int)(std::min_element(load, load + nst) - load);
             load[si] += cost(b->qcls[c]);
             cstream[c] = si;
@@ -1281,11 +1518,24 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         a.bt_mol = qc.bt_mol;
         a.bt_off = qc.bt_off;
         a.nbt = qc.nbt;
-        a.diag = qc.X == qc.Y;
+        a.diag = qc.GX == qc.GY;
         a.counters = tally ? b->counters + 4 * c : nullptr;   // work tallies only on request
+        if (qc.mode == 2) {
+            // merged classes: the merged row kernel (its LDS fit was checked for this build)
+            a.rt = qc.rt;
+            a.rr = qc.rr;
+            a.rtp = qc.rtp;
+            a.rprim = qc.rprim;
+            if (launch_jk_rows_gc(qc.la, qc.lb, qc.lc, qc.ld, qc.ca, qc.cb, qc.cc, qc.cd, qc.nrt, a,
+                                  row_shmem(qc, n_j, n_k, b->maxld), st))
+                return fail(ctx, MJK_EDEVICE, "merged row kernel launch failed for class (%d%d|%d%d)", qc.la, qc.lb,
+                            qc.lc, qc.ld);
+            if (prof) b->prof_kind[c] = 3;
+            if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c + 1], st));
+            continue;
+        }
         // row kernel (one block per bra pair, K' rows in LDS) when the rows fit
-        const size_t shmem = sizeof(double) * ((size_t)n_k * (ncart(qc.la) + ncart(qc.lb)) * b->maxld +
-                                               (size_t)n_j * ncart(qc.la) * ncart(qc.lb));
+        const size_t shmem = row_shmem(qc, n_j, n_k, b->maxld);
         int rr = -1;   // -1: no row kernel for this class (cooperative classes)
         if (shmem <= ctx->row_lds && qc.nrt > 0) {
             a.rt = qc.rt;
@@ -1374,8 +1624,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
             HIPCHK(ctx, hipMemcpy(cnt.data(), b->counters, sizeof(unsigned long long) * cnt.size(),
                                   hipMemcpyDeviceToHost));
         for (size_t c = 0; c < b->qcls.size(); ++c) {
-            if (b->qcls[c].nblocks == 0) continue;
-            if (b->class_filter >= 0 && (int)c != b->class_filter) continue;
+            if (!run[c]) continue;
             float ms = 0;
             HIPCHK(ctx, hipEventElapsedTime(&ms, b->pev[2 * c], b->pev[2 * c + 1]));
             b->prof_ms[c] += ms;
@@ -1402,8 +1651,11 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         stats->flops_fp32 = 0;
         for (size_t c = 0; c < b->qcls.size(); ++c) {
             const QClass& qc = b->qcls[c];
-            const double f64 = flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)cnt[4 * c], (double)cnt[4 * c + 1]);
-            const double f32 = flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)cnt[4 * c + 2], (double)cnt[4 * c + 3]);
+            const int nco = qc.ca * qc.cb * qc.cc * qc.cd;
+            const double f64 =
+                flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)cnt[4 * c], (double)cnt[4 * c + 1], nco);
+            const double f32 =
+                flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)cnt[4 * c + 2], (double)cnt[4 * c + 3], nco);
             stats->quartets += (long long)(cnt[4 * c] + cnt[4 * c + 2]);
             stats->prim_quartets += (long long)(cnt[4 * c + 1] + cnt[4 * c + 3]);
             stats->quartets_fp32 += (long long)cnt[4 * c + 2];
@@ -1492,20 +1744,27 @@ int mjk_basis_nclasses(const mjk_basis* b)
     return b ? (int)b->qcls.size() : MJK_EINVAL;
 }
 
-int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* b, int k, int n_dm, int hermi, int* l4, long long* nblocks,
-                    int* kernel)
+int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* b, int k, int n_dm, int hermi, int* l4, int* ctr4,
+                    long long* nblocks, int* kernel)
 {
-    if (!ctx || !b || b->ctx != ctx || k < 0 || k >= (int)b->qcls.size() || !l4 || !nblocks || !kernel)
+    if (!ctx || !b || b->ctx != ctx || k < 0 || k >= (int)b->qcls.size() || !l4 || !ctr4 || !nblocks || !kernel ||
+        n_dm <= 0)
         return fail(ctx, MJK_EINVAL, "bad arguments to mjk_basis_class");
     const QClass& qc = b->qcls[k];
     l4[0] = qc.la, l4[1] = qc.lb, l4[2] = qc.lc, l4[3] = qc.ld;
+    ctr4[0] = qc.ca, ctr4[1] = qc.cb, ctr4[2] = qc.cc, ctr4[3] = qc.cd;
     *nblocks = qc.nblocks;
-    // the choice the build makes for J and K of n_dm densities
+    // the choice a build of J and K of n_dm densities makes
     const int n_j = n_dm, n_k = hermi == 1 ? n_dm : 2 * n_dm;
-    const size_t shmem = sizeof(double) * ((size_t)n_k * (ncart(qc.la) + ncart(qc.lb)) * b->maxld +
-                                           (size_t)n_j * ncart(qc.la) * ncart(qc.lb));
-    const bool row = shmem <= ctx->row_lds && qc.nrt > 0 && row_kernel_available(qc.la, qc.lb, qc.lc, qc.ld);
-    *kernel = row ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
+    const std::vector<char> run = classes_to_run(ctx, b, n_j, n_k);
+    if (!run[k]) {
+        *kernel = -1;
+    } else if (qc.mode == 2) {
+        *kernel = 3;
+    } else {
+        const bool row = row_shmem(qc, n_j, n_k, b->maxld) <= ctx->row_lds && qc.nrt > 0;
+        *kernel = row ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
+    }
     return MJK_OK;
 }
 
@@ -1548,7 +1807,8 @@ int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* b, int max_classes, int* cls
         ms[k] = b->prof_ms[c];
         nq[k] = b->prof_nq[c];
         nprimq[k] = b->prof_np[c];
-        flops[k] = flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)b->prof_nq[c], (double)b->prof_np[c]);
+        flops[k] = flops_class(qc.la, qc.lb, qc.lc, qc.ld, (double)b->prof_nq[c], (double)b->prof_np[c],
+                               qc.ca * qc.cb * qc.cc * qc.cd);
         launches[k] = b->prof_launches[c];
         ++k;
     }

@@ -59,8 +59,8 @@ def _declare(lib):
     lib.mjk_basis_fetch_jk.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_void_p]
     lib.mjk_basis_set_class_filter.argtypes = [c_void_p, c_void_p, c_int]
     lib.mjk_basis_nclasses.argtypes = [c_void_p]
-    lib.mjk_basis_class.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, P(ctypes.c_longlong),
-                                    P(c_int)]
+    lib.mjk_basis_class.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                                    P(ctypes.c_longlong), P(c_int)]
     lib.mjk_device_alloc.argtypes = [c_void_p, c_size_t, P(c_void_p)]
     lib.mjk_device_free.argtypes = [c_void_p, c_void_p]
     lib.mjk_memcpy_h2d.argtypes = [c_void_p, c_void_p, c_void_p, c_size_t]

@@ -196,16 +196,19 @@ class Basis:
                           'mjk_basis_set_class_filter')
 
     def classes(self, n_dm=1, hermi=1):
-        """[(index, (la, lb, lc, ld), nblocks, kernel)] of the class list;
-        kernel 0 tile, 1 row, 2 cooperative."""
+        """[(index, (la, lb, lc, ld), (ca, cb, cc, cd), nblocks, kernel)] of the
+        class list; ctr = contraction counts of the shells (2: a generally
+        contracted s shell kept whole); kernel 0 tile, 1 row, 2 cooperative,
+        3 merged row kernel, -1 the class does not run in such a build."""
         L = self.engine.lib
         out = []
         l4 = numpy.zeros(4, dtype=numpy.int32)
+        c4 = numpy.zeros(4, dtype=numpy.int32)
         nb, kern = ctypes.c_longlong(), ctypes.c_int()
         for k in range(L.mjk_basis_nclasses(self.handle)):
-            self.engine.check(L.mjk_basis_class(self.engine.handle, self.handle, k, n_dm, hermi, _p(l4),
+            self.engine.check(L.mjk_basis_class(self.engine.handle, self.handle, k, n_dm, hermi, _p(l4), _p(c4),
                                                 ctypes.byref(nb), ctypes.byref(kern)), 'mjk_basis_class')
-            out.append((k, tuple(int(x) for x in l4), int(nb.value), int(kern.value)))
+            out.append((k, tuple(int(x) for x in l4), tuple(int(x) for x in c4), int(nb.value), int(kern.value)))
         return out
 
 

@@ -53,11 +53,11 @@ def test_class_builds_add_up(water4, fp32_tol):
     full = b.get_jk_resident(1, 1, True, True, SCF_TOL, fp32_tol, stats=True)
     sj, sk = numpy.zeros_like(vj), numpy.zeros_like(vk)
     nq = nq32 = 0
-    classes = [c for c in b.classes() if c[2] > 0]
+    classes = [c for c in b.classes() if c[3] > 0 and c[4] >= 0]
     assert len(classes) >= 10
     try:
-        for k, l4, nblocks, kernel in classes:
-            assert kernel in (0, 1, 2)
+        for k, l4, ctr, nblocks, kernel in classes:
+            assert kernel in (0, 1, 2, 3)
             b.set_class_filter(k)
             st = b.get_jk_resident(1, 1, True, True, SCF_TOL, fp32_tol, stats=True)
             cj, ck = b.fetch_jk(1)
