@@ -172,6 +172,31 @@ int mjk_basis_nclasses(const mjk_basis* basis);
 int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* basis, int k, int n_dm, int hermi, int* l4, int* ctr4,
                     long long* nblocks, int* kernel);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The build of mjk_get_jk_device (same kernels, results
+ * equal to rounding) with every work buffer -- Cartesian densities, the
+ * private J'/K' accumulator copies, dm_cond -- carved out of a static device
+ * array of MJK_NOALLOC_SCRATCH_BYTES per device, so that the call allocates
+ * and frees no device memory.  The caller passes the inputs (a basis handle,
+ * the densities in device memory) and the outputs (device memory).
+ *   - The basis's quartet task lists must exist for direct_scf_tol: upload it
+ *     with that cutoff (or run one mjk_get_jk_device build at it); otherwise
+ *     MJK_EINVAL.
+ *   - As many accumulator copies as fit are used (at most the context's);
+ *     a build whose densities, one J'/K' copy and dm_cond do not fit returns
+ *     MJK_ENOMEM (e.g. one density of about 2900 Cartesian functions).
+ *   - No statistics, no profiling, no graph replay; asynchronous on the
+ *     context's stream.
+ *   - Every call on one device shares that device's array, which keeps the
+ *     last call's contents (each call clears what must start at zero): calls
+ *     that overlap on one GPU -- from two contexts, or streams not ordered
+ *     with each other -- race.  Calls on one context are ordered. */
+#define MJK_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int mjk_get_jk_device_noalloc(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n_dm, int hermi,
+                              int with_j, int with_k, double direct_scf_tol, double fp32_tol,
+                              double* d_vj, double* d_vk);
+
 /* Per-quartet-class kernel profiling with HIP events on the launch stream
  * (the classes of a build run concurrently on up to four streams, so the
  * per-class times of one build overlap).

@@ -296,6 +296,45 @@ void dfree(mjk_ctx* ctx, void* p)
     free_now(ctx->device, p);
 }
 
+// Work buffers of the no-allocation build (mjk_get_jk_device_noalloc): one
+// array per device, resident from the library's load to the process's end.
+// In host code the name `scratch` is a host-side stand-in and never a device
+// address: each device's address is taken once with hipGetSymbolAddress while
+// that device is current (mjk_ctx_create) and kept in g_scratch_addr, and only
+// that address is passed to kernels.
+static __device__ __align__(256) double scratch[MJK_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[MAX_DEVICES] = {};
+std::string g_scratch_err[MAX_DEVICES];
+
+// Called with `device` current.
+void scratch_init(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_addr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+        return;
+    }
+    g_scratch_addr[device] = static_cast<double*>(p);
+}
+
+double* scratch_address(int device, std::string* why)
+{
+    if (device < 0 || device >= MAX_DEVICES) {
+        *why = "device index out of range";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[device])
+        *why = g_scratch_err[device].empty() ? "scratch not initialised" : g_scratch_err[device];
+    return g_scratch_addr[device];
+}
+
 template <typename T>
 int upload(mjk_ctx* ctx, const std::vector<T>& v, T** d)
 {
@@ -1239,6 +1278,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
         mjk_ctx_destroy(ctx);
         return MJK_EDEVICE;
     }
+    scratch_init(device);   // once per device; a failure is reported by the no-allocation build
     std::vector<double> tab(RYS_BUF_SIZE);
     rys_fill_buffer(tab.data());
     if (upload(ctx, tab, &ctx->rys) || upload_c2s(ctx->stream) || hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -1399,9 +1439,15 @@ int mjk_basis_q_cond(mjk_ctx* ctx, const mjk_basis* b, int mol, double* q_cond)
     return MJK_OK;
 }
 
-int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm, int hermi, int with_j,
-                      int with_k, double direct_scf_tol, double fp32_tol, double* d_vj, double* d_vk,
-                      mjk_stats* stats)
+// One J/K build.  scratch == nullptr: the basis's own work buffers (grown on
+// demand), task lists rebuilt when the cutoff changes, graph replay, profiling
+// and statistics.  scratch != nullptr (this device's address of the
+// no-allocation scratch array): the work buffers are carved out of it and no
+// device memory is allocated or freed -- the task lists must already exist
+// for this cutoff, and the build runs eagerly without statistics or profiling.
+static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm, int hermi, int with_j,
+                       int with_k, double direct_scf_tol, double fp32_tol, double* d_vj, double* d_vk,
+                       mjk_stats* stats, double* scratch)
 {
     if (!ctx || !b || b->ctx != ctx) return fail(ctx, MJK_EINVAL, "context/basis mismatch");
     if (n_dm <= 0 || !d_dms) return fail(ctx, MJK_EINVAL, "n_dm must be positive");
@@ -1411,42 +1457,73 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     std::lock_guard<std::recursive_mutex> lock(ctx->mu);
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    drain_deferred();
+    const bool noalloc = scratch != nullptr;
+    if (!noalloc) drain_deferred();
     const bool screen = direct_scf_tol > 0;
     const double cut = screen ? direct_scf_tol : 0.0;
     int rc;
-    if (b->tile_cutoff != cut)
+    if (b->tile_cutoff != cut) {
+        if (noalloc)
+            return fail(ctx, MJK_EINVAL,
+                        "no-allocation build: the basis's quartet task lists are built for direct_scf_tol %g, "
+                        "not %g (upload the basis with this cutoff, or run one mjk_get_jk_device build at it)",
+                        b->tile_cutoff, cut);
         if ((rc = build_tiles(b, cut))) return rc;
+    }
     const int n_j = with_j ? n_dm : 0;
     const int n_k = with_k ? (hermi == 1 ? n_dm : 2 * n_dm) : 0;
-    if ((rc = ensure(ctx, &b->Dj, b->cap_dj, (long long)std::max(n_j, 1) * b->slot_c))) return rc;
-    const int ncopy = ctx->ncopy;
     const long long jcopy = (long long)std::max(n_j, 1) * b->slot_c;
     const long long kcopy = (long long)std::max(n_k, 1) * b->slot_c;
-    if ((rc = ensure(ctx, &b->J, b->cap_j, ncopy * jcopy))) return rc;
-    if ((rc = ensure(ctx, &b->Dk, b->cap_dk, (long long)std::max(n_k, 1) * b->slot_c))) return rc;
-    if ((rc = ensure(ctx, &b->K, b->cap_k, ncopy * kcopy))) return rc;
     const bool need_cond = screen || fp32_tol > 0;
-    if (need_cond && (rc = ensure(ctx, &b->dmcond, b->cap_cond, b->cond_total))) return rc;
+    // work buffers: Cartesian densities, private J'/K' copies, dm_cond
+    double *Dj, *Dk, *Jacc, *Kacc, *dmcond;
+    int ncopy = ctx->ncopy;
+    if (!noalloc) {
+        if ((rc = ensure(ctx, &b->Dj, b->cap_dj, jcopy))) return rc;
+        if ((rc = ensure(ctx, &b->J, b->cap_j, ncopy * jcopy))) return rc;
+        if ((rc = ensure(ctx, &b->Dk, b->cap_dk, kcopy))) return rc;
+        if ((rc = ensure(ctx, &b->K, b->cap_k, ncopy * kcopy))) return rc;
+        if (need_cond && (rc = ensure(ctx, &b->dmcond, b->cap_cond, b->cond_total))) return rc;
+        Dj = b->Dj, Dk = b->Dk, Jacc = b->J, Kacc = b->K, dmcond = b->dmcond;
+    } else {
+        // each buffer starts on a 256-byte boundary; as many accumulator copies
+        // as fit (at most the context's), at least one
+        auto up = [](long long n) { return (n + 31) / 32 * 32; };
+        const long long cap = (long long)(MJK_NOALLOC_SCRATCH_BYTES / sizeof(double));
+        auto need = [&](int nc) {
+            return up(jcopy) + up(kcopy) + up(nc * jcopy) + up(nc * kcopy) + (need_cond ? up(b->cond_total) : 0);
+        };
+        while (ncopy > 1 && need(ncopy) > cap) --ncopy;
+        if (need(ncopy) > cap)
+            return fail(ctx, MJK_ENOMEM,
+                        "no-allocation build needs %lld bytes of scratch; its static scratch holds %lld bytes",
+                        need(1) * (long long)sizeof(double), cap * (long long)sizeof(double));
+        double* q = scratch;
+        Dj = q, q += up(jcopy);
+        Dk = q, q += up(kcopy);
+        Jacc = q, q += up(ncopy * jcopy);
+        Kacc = q, q += up(ncopy * kcopy);
+        dmcond = need_cond ? q : nullptr;
+    }
     hipStream_t st = ctx->stream;
     std::vector<char> run = classes_to_run(ctx, b, n_j, n_k);
     if (b->class_filter >= 0)
         for (size_t c = 0; c < run.size(); ++c) run[c] = run[c] && (int)c == b->class_filter;
-    const bool prof = b->profiling && b->pev.size() >= 2 * b->qcls.size();
-    const bool tally = stats || (prof && b->prof_counts);
+    const bool prof = !noalloc && b->profiling && b->pev.size() >= 2 * b->qcls.size();
+    const bool tally = !noalloc && (stats || (prof && b->prof_counts));
     // The whole build on the context's stream and the streams it forks.
     // capture: recorded into a graph, without the timing events.  The
     // accumulators are cleared by a kernel, not a memset node.
     auto enqueue = [&](bool capture) -> int {
     if (!capture) HIPCHK(ctx, hipEventRecord(ctx->ev[0], st));
-    if (n_j && launch_zero(b->J, ncopy * jcopy, st)) return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
-    if (n_k && launch_zero(b->K, ncopy * kcopy, st)) return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
+    if (n_j && launch_zero(Jacc, ncopy * jcopy, st)) return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
+    if (n_k && launch_zero(Kacc, ncopy * kcopy, st)) return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
     // (the 64-bit counters are cleared as doubles: both are all-zero bits)
     if (tally && launch_zero(reinterpret_cast<double*>(b->counters), 4 * NQCLASS_MAX, st))
         return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
-    if (need_cond && launch_dmcond(dmcond_args(b), d_dms, b->slot_s, n_dm, b->dmcond, st))
+    if (need_cond && launch_dmcond(dmcond_args(b), d_dms, b->slot_s, n_dm, dmcond, st))
         return fail(ctx, MJK_EDEVICE, "dm_cond launch failed");
-    if (launch_sph2cart(xform(b), b->maxld, d_dms, b->slot_s, n_dm, hermi, with_j, with_k, b->Dj, b->Dk,
+    if (launch_sph2cart(xform(b), b->maxld, d_dms, b->slot_s, n_dm, hermi, with_j, with_k, Dj, Dk,
                         b->slot_c, st))
         return fail(ctx, MJK_EDEVICE, "density transform launch failed");
     if (!capture) HIPCHK(ctx, hipEventRecord(ctx->ev[1], st));
@@ -1456,17 +1533,17 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     a.M.ld = b->d_ld;
     a.M.cbase = b->d_cbase;
     a.M.nsh = b->d_nsh;
-    a.Dj = b->Dj;
-    a.Dk = b->Dk;
-    a.J = b->J;
-    a.K = b->K;
+    a.Dj = Dj;
+    a.Dk = Dk;
+    a.J = Jacc;
+    a.K = Kacc;
     a.slot = b->slot_c;
     a.n_j = n_j;
     a.n_k = n_k;
     a.ncopy = ncopy;
     a.jcopy = jcopy;
     a.kcopy = kcopy;
-    a.dm_cond = b->dmcond;
+    a.dm_cond = dmcond;
     a.cutoff = cut;
     a.screen = screen ? 1 : 0;
     a.fp32_tol = fp32_tol > 0 ? fp32_tol : 0.0;
@@ -1555,7 +1632,7 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
         HIPCHK(ctx, hipStreamWaitEvent(st, ctx->ev_join[i - 1], 0));
     }
     if (!capture) HIPCHK(ctx, hipEventRecord(ctx->ev[2], st));
-    if (launch_cart2sph(xform(b), b->maxnao, b->J, b->K, b->slot_c, n_dm, hermi, with_j, with_k, d_vj, d_vk,
+    if (launch_cart2sph(xform(b), b->maxnao, Jacc, Kacc, b->slot_c, n_dm, hermi, with_j, with_k, d_vj, d_vk,
                         b->slot_s, ncopy, jcopy, kcopy, st))
         return fail(ctx, MJK_EDEVICE, "output transform launch failed");
     if (!capture) HIPCHK(ctx, hipEventRecord(ctx->ev[3], st));
@@ -1564,12 +1641,12 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
     // Plain builds (no statistics, no profiling) replay one HIP graph of the
     // whole build while nothing it depends on changes: one launch instead of
     // one per kernel (the launch-bound small molecules gain most).
-    if (ctx->graphs && !prof && !tally) {
+    if (ctx->graphs && !prof && !tally && !noalloc) {
         GraphKey key;
         key.dms = d_dms;
         key.vj = with_j ? d_vj : nullptr;
         key.vk = with_k ? d_vk : nullptr;
-        const void* bufs[5] = {b->J, b->K, b->Dj, b->Dk, b->dmcond};
+        const void* bufs[5] = {Jacc, Kacc, Dj, Dk, dmcond};
         for (int i = 0; i < 5; ++i) key.bufs[i] = bufs[i];
         key.n_dm = n_dm;
         key.hermi = hermi;
@@ -1666,6 +1743,25 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm,
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return MJK_OK;
+}
+
+int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm, int hermi, int with_j,
+                      int with_k, double direct_scf_tol, double fp32_tol, double* d_vj, double* d_vk,
+                      mjk_stats* stats)
+{
+    return get_jk_impl(ctx, b, d_dms, n_dm, hermi, with_j, with_k, direct_scf_tol, fp32_tol, d_vj, d_vk, stats,
+                       nullptr);
+}
+
+int mjk_get_jk_device_noalloc(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm, int hermi, int with_j,
+                              int with_k, double direct_scf_tol, double fp32_tol, double* d_vj, double* d_vk)
+{
+    if (!ctx) return MJK_EINVAL;
+    std::string why;
+    double* ptr = scratch_address(ctx->device, &why);
+    if (!ptr) return fail(ctx, MJK_EDEVICE, "no scratch on device %d: %s", ctx->device, why.c_str());
+    return get_jk_impl(ctx, b, d_dms, n_dm, hermi, with_j, with_k, direct_scf_tol, fp32_tol, d_vj, d_vk, nullptr,
+                       ptr);
 }
 
 int mjk_get_jk(mjk_ctx* ctx, mjk_basis* b, const double* dms, int n_dm, int hermi, int with_j, int with_k,

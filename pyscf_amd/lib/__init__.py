@@ -54,6 +54,7 @@ def _declare(lib):
     lib.mjk_get_jk.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                c_void_p, c_void_p, P(mjk_stats)]
     lib.mjk_get_jk_device.argtypes = lib.mjk_get_jk.argtypes
+    lib.mjk_get_jk_device_noalloc.argtypes = lib.mjk_get_jk.argtypes[:-1]
     lib.mjk_get_jk_resident.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                         P(mjk_stats)]
     lib.mjk_basis_fetch_jk.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_void_p]
