@@ -169,8 +169,132 @@ int mjk_get_jk_resident(mjk_ctx* ctx, mjk_basis* basis, int n_dm, int hermi, int
 int mjk_basis_fetch_jk(mjk_ctx* ctx, mjk_basis* basis, int n_dm, double* vj, double* vk);
 int mjk_basis_set_class_filter(mjk_ctx* ctx, mjk_basis* basis, int cls);
 int mjk_basis_nclasses(const mjk_basis* basis);
+/* Tuning key of class k: key9 = (la lb lc ld ca cb cc cd bucket), bucket =
+ * floor(log2) of the class's Schwarz-prescreened primitive quartets. */
+int mjk_basis_class_key(mjk_ctx* ctx, const mjk_basis* basis, int k, int* key9);
+/* Per-class tuning table (no reference counterpart; the reference has one CPU
+ * code path with fixed 64-AO blocking, pyscf/lib/vhf/nr_direct.c:316-340).
+ * n entries of 11 ints (la lb lc ld ca cb cc cd bucket kernel rprim) and one
+ * double (scale): a class takes the entry of its key with the nearest bucket
+ * (at most 2 away) -- kernel 0 the default choice, 1 the row kernel, 2 its
+ * fallback (a merged class then runs its split pairs); rprim > 0 bra
+ * primitive pairs per row task; scale > 0 multiplies the row tasks' size.
+ * Replaces the table; every basis rebuilds its task lists at its next build.
+ * Results agree with the untuned build to rounding. */
+int mjk_ctx_set_tuning(mjk_ctx* ctx, int n, const int* keys, const double* scale);
+/* Entries of the context's tuning table.  mjk_ctx_create loads the table
+ * from the file MJK_TUNING names, else tuning/mi355x.txt beside the
+ * library's folder (lines "la lb lc ld ca cb cc cd bucket kernel rprim
+ * scale ...", '#' comments); MJK_TUNE=0 starts with none. */
+int mjk_ctx_tuning_entries(mjk_ctx* ctx);
 int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* basis, int k, int n_dm, int hermi, int* l4, int* ctr4,
                     long long* nblocks, int* kernel);
+
+/* Device-resident closed-shell SCF: the RHF loop of pyscf/scf/hf.py:47-242
+ * (kernel; get_fock/CDIIS pyscf/scf/diis.py:40-66; eig :1340-1347; make_rdm1
+ * :840; incremental get_veff :2416-2427) with the density, Fock matrix, DIIS
+ * history and MO coefficients in HBM, J/K from this engine and the dense
+ * algebra on rocBLAS/rocSOLVER on the context's stream.  The reference runs
+ * this loop in numpy with D crossing to the C driver every cycle.
+ *   mjk_rhf_create: basis = one molecule; s1e/h1e = overlap and core
+ *     Hamiltonian (host, nao x nao); nocc doubly occupied orbitals; enuc =
+ *     nuclear repulsion; diis_space vectors (8 in the reference).
+ *   mjk_rhf_run: dm0 = initial density (host) or NULL for the core
+ *     Hamiltonian guess; converged when |dE| < conv_tol and the orbital
+ *     gradient norm < conv_tol_grad; level_shift shifts the virtual space
+ *     after DIIS (0: none).  Synchronous; fills res.
+ *   mjk_rhf_fetch: any of dm, fock (nao x nao), mo_energy (nao), mo_coeff
+ *     (row-major nao x nmo) may be NULL.
+ *   mjk_rhf_device_density: the converged density in device memory (valid
+ *     until mjk_rhf_destroy). */
+typedef struct mjk_scf mjk_scf;
+typedef struct {
+    double e_tot;
+    double gnorm;
+    int cycles;
+    int converged;
+} mjk_rhf_result;
+int mjk_rhf_create(mjk_ctx* ctx, mjk_basis* basis, const double* s1e, const double* h1e, int nocc, double enuc,
+                   int diis_space, mjk_scf** scf);
+int mjk_rhf_run(mjk_ctx* ctx, mjk_scf* scf, const double* dm0, int max_cycle, double conv_tol, double conv_tol_grad,
+                double direct_scf_tol, double fp32_tol, double level_shift, mjk_rhf_result* res);
+int mjk_rhf_fetch(mjk_ctx* ctx, mjk_scf* scf, double* dm, double* mo_energy, double* mo_coeff, double* fock);
+const double* mjk_rhf_device_density(const mjk_scf* scf);
+void mjk_rhf_destroy(mjk_scf* scf);
+
+/* Request server (continuous batching on one device).  The reference serves
+ * one caller at a time: each SCF.get_jk call crosses into CVHFnr_direct_drv
+ * once (pyscf/scf/_vhf.py:540-639) and returns.  Here requests -- a J/K build
+ * of given densities, or a whole RHF -- are submitted, run concurrently on
+ * nslots worker contexts (one stream each), and are collected by ticket.
+ *   mjk_server_create: hbm_budget = bytes the running requests may hold
+ *     together (0: 80% of the free device memory).
+ *   mjk_submit: copies the request (molecule arrays, densities); returns a
+ *     ticket at once.  MJK_ENOMEM when the request's estimate alone exceeds
+ *     the budget.  Admission is in arrival order: the head starts when a
+ *     slot is free and the bytes the running requests' contexts have tracked
+ *     (each at least its estimate until its buffers exist) plus the head's
+ *     estimate fit the budget.  An RHF request runs the device-resident SCF
+ *     (mjk_rhf_*: core guess, CDIIS 8, converged at conv_tol and
+ *     sqrt(conv_tol) on the orbital gradient).
+ *   mjk_poll: state and timings of a ticket, non-blocking.
+ *   mjk_wait_any: waits up to timeout_s for a request to finish; *ticket =
+ *     the next finished ticket not yet returned (0 on time-out).
+ *   mjk_result: copies the outputs (J/K: n_dm x nao x nao each; RHF: the
+ *     density and orbital energies; any pointer may be NULL) and releases the
+ *     request; a failed request returns its error.
+ *   mjk_server_hbm_held: the bytes admission counts now.
+ *   mjk_server_error: the calling thread's last server error.
+ *   mjk_server_destroy: runs the queued requests to the end, then frees. */
+typedef struct mjk_server mjk_server;
+#define MJK_REQ_JK 0
+#define MJK_REQ_RHF 1
+#define MJK_REQ_QUEUED 0
+#define MJK_REQ_RUNNING 1
+#define MJK_REQ_DONE 2
+#define MJK_REQ_FAILED 3
+typedef struct {
+    int kind;                 /* MJK_REQ_JK or MJK_REQ_RHF */
+    const int* atm;           /* natm x 6, pyscf layout */
+    int natm;
+    const int* bas;           /* nbas x 8 */
+    int nbas;
+    const double* env;
+    int env_size;
+    double direct_scf_tol;
+    double fp32_tol;
+    /* MJK_REQ_JK */
+    const double* dm;         /* n_dm x nao x nao */
+    int n_dm;
+    int hermi;
+    int with_j;
+    int with_k;
+    /* MJK_REQ_RHF */
+    int nelectron;
+    int max_cycle;
+    double conv_tol;
+} mjk_request;
+typedef struct {
+    int state;                /* MJK_REQ_* */
+    int nao;
+    int n_dm;
+    int cycles;
+    int converged;
+    int slot;
+    double e_tot;
+    double t_submit, t_start, t_done;   /* seconds since the server's creation */
+    size_t hbm_estimate;
+    size_t hbm_peak;          /* tracked device bytes the request held at most */
+    char error[256];
+} mjk_request_info;
+int mjk_server_create(int device, int nslots, size_t hbm_budget, mjk_server** server);
+int mjk_submit(mjk_server* server, const mjk_request* request, long long* ticket);
+int mjk_poll(mjk_server* server, long long ticket, mjk_request_info* info);
+int mjk_wait_any(mjk_server* server, double timeout_s, long long* ticket);
+int mjk_result(mjk_server* server, long long ticket, double* vj, double* vk, double* dm, double* mo_energy);
+size_t mjk_server_hbm_held(mjk_server* server);
+const char* mjk_server_error(const mjk_server* server);
+void mjk_server_destroy(mjk_server* server);
 
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that

@@ -21,6 +21,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
+#include <atomic>
+#include <fstream>
+#include <sstream>
+#include <dlfcn.h>
 #include <map>
 #include <tuple>
 #include <mutex>
@@ -31,6 +36,7 @@
 #include <vector>
 
 #include "mjk.h"
+#include "mjk_internal.h"
 #include "c2s.h"
 #include "jk_common.h"
 #include "rys_eval.h"
@@ -148,6 +154,8 @@ struct QClass {
     int2* rr = nullptr;
     int* rtp = nullptr;      // bra primitive offset of every task
     int rprim = 0;           // bra primitive pairs per task
+    int bucket = 0;          // floor(log2) of the Schwarz-prescreened primitive quartets (tuning key)
+    int kernel_pref = 0;     // tuning: 0 the default choice, 1 row kernel, 2 fallback (tile / cooperative)
 };
 
 
@@ -189,8 +197,16 @@ struct mjk_ctx {
     int row_blocks = 16384;   // row-kernel blocks per class launch to aim for ...
     int row_min_blocks = 512;    // ... and the fewest, whatever row_chunk asks
     int row_prim = 0;            // bra primitive pairs per row task (0: by class size)
+    // per-class tuning table (mjk_ctx_set_tuning): key (la lb lc ld ca cb cc cd bucket) ->
+    // {kernel preference, bra primitive pairs per task, task-size scale}; tune_gen counts
+    // its changes, and a basis rebuilds its task lists when it sees a new generation
+    std::map<std::array<int, 9>, std::tuple<int, int, double>> tune;
+    int tune_gen = 0;
     hipStream_t stream = nullptr;
-    size_t budget = 0, used = 0;
+    void* blas = nullptr;                 // dense-algebra handle of the device-resident SCF
+    void (*blas_destroy)(void*) = nullptr;
+    size_t budget = 0;
+    std::atomic<size_t> used{0};  // tracked device bytes (read by request admission on other threads)
     std::string err;
     double* rys = nullptr;
     std::map<void*, size_t> allocs;
@@ -234,7 +250,7 @@ int dalloc(mjk_ctx* ctx, size_t bytes, void** p)
     if (bytes == 0) return MJK_OK;
     if (ctx->used + bytes > ctx->budget)
         return fail(ctx, MJK_ENOMEM, "HBM budget exceeded: need %zu more bytes, %zu of %zu in use", bytes,
-                    ctx->used, ctx->budget);
+                    ctx->used.load(), ctx->budget);
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return fail(ctx, MJK_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     ctx->used += bytes;
@@ -400,6 +416,7 @@ struct mjk_basis {
     int* d_pmol = nullptr;           // molecule of every pair
     int* d_active = nullptr;         // per molecule: 0 skips its quartets (mjk_basis_set_active)
     int class_filter = -1;           // >= 0: builds run only this quartet class (mjk_basis_set_class_filter)
+    int tune_gen = -1;               // ctx->tune_gen the task lists were built with
     std::vector<double> q;           // Schwarz bound per pair (sorted order)
     std::vector<int> h_shA, h_shB, h_nprim;
     std::vector<long long> pcost;    // prefix sums of h_nprim (row-kernel task sizes)
@@ -451,6 +468,24 @@ void free_tiles(mjk_basis* b)
     }
     b->qcls.clear();
     b->tile_cutoff = -1.0;
+}
+
+// Tuning entry of a class: the entry of its (angular momenta, contraction
+// pattern) with the nearest work bucket, at most 2 buckets away; null if none.
+const std::tuple<int, int, double>* find_tune(const mjk_ctx* ctx, const QClass& q)
+{
+    const std::tuple<int, int, double>* best = nullptr;
+    int dist = 3;
+    for (int d = 0; d <= 2 && !best; ++d)
+        for (int sgn : {1, -1}) {
+            const std::array<int, 9> key = {q.la, q.lb, q.lc, q.ld, q.ca, q.cb, q.cc, q.cd, q.bucket + sgn * d};
+            auto it = ctx->tune.find(key);
+            if (it != ctx->tune.end() && d < dist) {
+                best = &it->second;
+                dist = d;
+            }
+        }
+    return best;
 }
 
 // Bra tiles of TILE pairs; each keeps the ket prefix whose Schwarz product
@@ -524,6 +559,9 @@ int build_tiles(mjk_basis* b, double cutoff)
                   ((qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1);
         qc.est = flops_class(qc.la, qc.lb, qc.lc, qc.ld, est_q * qc.ca * qc.cb * qc.cc * qc.cd, est_p,
                              qc.ca * qc.cb * qc.cc * qc.cd);
+        qc.bucket = est_p >= 1.0 ? (int)std::floor(std::log2(est_p)) : 0;
+        const std::tuple<int, int, double>* tuned = find_tune(ctx, qc);
+        qc.kernel_pref = tuned ? std::get<0>(*tuned) : 0;
         if (qc.nbt > 0) {
             int rc;
             if ((rc = upload(ctx, bt, &qc.bt))) return rc;
@@ -569,8 +607,10 @@ int build_tiles(mjk_basis* b, double cutoff)
             }
             // enough blocks to fill the chip, each long enough to amortise its
             // fixed cost (LDS clear, range set-up, row flush)
-            const long long budget = std::min(std::max(cost_all / ctx->row_blocks, (long long)ctx->row_chunk),
-                                              std::max(cost_all / ctx->row_min_blocks, 1LL));
+            long long budget = std::min(std::max(cost_all / ctx->row_blocks, (long long)ctx->row_chunk),
+                                        std::max(cost_all / ctx->row_min_blocks, 1LL));
+            if (tuned && std::get<2>(*tuned) > 0)
+                budget = std::max(1LL, (long long)((double)budget * std::get<2>(*tuned)));
             std::vector<std::pair<long long, int4>> tasks;
             std::vector<int> tprim;   // bra primitive offset of every task (parallel to tasks)
             // bra primitive pairs per task: small classes give every bra primitive
@@ -578,8 +618,11 @@ int build_tiles(mjk_basis* b, double cutoff)
             // critical path: H2O cc-pVDZ 0.69 -> 0.52 ms per build from 2 to 1);
             // large ones share them in eights (measured on MI355X, (H2O)16 def2-SVP
             // with the classes on four streams: 2 -> 92, 4 -> 107, 8 -> 110 builds/s)
-            const int P = ctx->row_prim > 0 ? ctx->row_prim
-                          : cost_all < (1LL << 18) ? 1 : cost_all < (1LL << 20) ? 2 : 8;
+            const int P = ctx->row_prim > 0                     ? ctx->row_prim
+                          : tuned && std::get<1>(*tuned) > 0 ? std::get<1>(*tuned)
+                          : cost_all < (1LL << 18)            ? 1
+                          : cost_all < (1LL << 20)            ? 2
+                                                              : 8;
             qc.rprim = P;
             for (size_t ib = 0; ib < bra.size(); ++ib)
             for (int pbo = 0; pbo < b->h_nprim[bra[ib]]; pbo += P) {
@@ -687,6 +730,7 @@ int build_tiles(mjk_basis* b, double cutoff)
         }
     }
     b->tile_cutoff = cutoff;
+    b->tune_gen = ctx->tune_gen;
     return MJK_OK;
 }
 
@@ -1216,7 +1260,8 @@ std::vector<char> classes_to_run(const mjk_ctx* ctx, const mjk_basis* b, int n_j
     for (const QClass& q : b->qcls)
         if (q.mode == 2) merged[q.base] = 1;
     for (const QClass& q : b->qcls)
-        if (q.mode == 2 && q.nblocks > 0 && (q.nrt == 0 || row_shmem(q, n_j, n_k, b->maxld) > ctx->row_lds))
+        if (q.mode == 2 && q.nblocks > 0 &&
+            (q.nrt == 0 || q.kernel_pref == 2 || row_shmem(q, n_j, n_k, b->maxld) > ctx->row_lds))
             merged[q.base] = 0;
     std::vector<char> run(b->qcls.size(), 0);
     for (size_t c = 0; c < b->qcls.size(); ++c) {
@@ -1239,6 +1284,80 @@ int ensure(mjk_ctx* ctx, double** p, long long& cap, long long need)
 }
 
 }  // namespace
+
+namespace {
+
+// The tuning table the context starts with: the file MJK_TUNING names, else
+// tuning/mi355x.txt beside the library's lib/ folder; none when MJK_TUNE=0.
+// Lines: la lb lc ld ca cb cc cd bucket kernel rprim scale [anything], '#'
+// starts a comment.  An unreadable or malformed file leaves the table empty.
+void load_tuning_file(mjk_ctx* ctx)
+{
+    if (const char* t = getenv("MJK_TUNE"))
+        if (atoi(t) == 0) return;
+    std::string path;
+    if (const char* f = getenv("MJK_TUNING")) {
+        path = f;
+    } else {
+        Dl_info info;
+        if (!dladdr((void*)&mjk_ctx_create, &info) || !info.dli_fname) return;
+        std::string lib = info.dli_fname;
+        const size_t slash = lib.find_last_of('/');
+        const std::string dir = slash == std::string::npos ? std::string(".") : lib.substr(0, slash);
+        path = dir + "/../tuning/mi355x.txt";
+    }
+    std::ifstream in(path);
+    if (!in) return;
+    std::vector<int> keys;
+    std::vector<double> scale;
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.resize(hash);
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        std::istringstream ls(line);
+        int k[11];
+        double sc;
+        for (int i = 0; i < 11; ++i)
+            if (!(ls >> k[i])) return;
+        if (!(ls >> sc)) return;
+        keys.insert(keys.end(), k, k + 11);
+        scale.push_back(sc);
+    }
+    if (mjk_ctx_set_tuning(ctx, (int)scale.size(), keys.data(), scale.data()) != MJK_OK) {
+        ctx->err.clear();
+        ctx->tune.clear();
+    }
+}
+
+}  // namespace
+
+namespace mjk_internal {
+
+hipStream_t ctx_stream(mjk_ctx* ctx) { return ctx->stream; }
+int ctx_device(const mjk_ctx* ctx) { return ctx->device; }
+size_t ctx_budget(const mjk_ctx* ctx) { return ctx->budget; }
+std::recursive_mutex& ctx_mutex(mjk_ctx* ctx) { return ctx->mu; }
+std::shared_mutex& capture_mutex() { return g_capture; }
+void* ctx_blas(mjk_ctx* ctx) { return ctx->blas; }
+void ctx_set_blas(mjk_ctx* ctx, void* handle, void (*destroy)(void*))
+{
+    ctx->blas = handle;
+    ctx->blas_destroy = destroy;
+}
+
+int ctx_fail(mjk_ctx* ctx, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    return code;
+}
+
+}  // namespace mjk_internal
 
 extern "C" {
 
@@ -1279,6 +1398,7 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
         return MJK_EDEVICE;
     }
     scratch_init(device);   // once per device; a failure is reported by the no-allocation build
+    load_tuning_file(ctx);
     std::vector<double> tab(RYS_BUF_SIZE);
     rys_fill_buffer(tab.data());
     if (upload(ctx, tab, &ctx->rys) || upload_c2s(ctx->stream) || hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -1309,6 +1429,10 @@ void mjk_ctx_destroy(mjk_ctx* ctx)
         if (ctx->ev_join[i]) (void)hipEventDestroy(ctx->ev_join[i]);
     }
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->blas && ctx->blas_destroy) {
+        std::unique_lock<std::shared_mutex> lk(g_capture);
+        ctx->blas_destroy(ctx->blas);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1320,7 +1444,7 @@ const char* mjk_last_error(const mjk_ctx* ctx)
 
 size_t mjk_ctx_hbm_used(const mjk_ctx* ctx)
 {
-    return ctx ? ctx->used : 0;
+    return ctx ? ctx->used.load() : 0;
 }
 
 int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes)
@@ -1462,12 +1586,13 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
     const bool screen = direct_scf_tol > 0;
     const double cut = screen ? direct_scf_tol : 0.0;
     int rc;
-    if (b->tile_cutoff != cut) {
+    if (b->tile_cutoff != cut || b->tune_gen != ctx->tune_gen) {
         if (noalloc)
             return fail(ctx, MJK_EINVAL,
-                        "no-allocation build: the basis's quartet task lists are built for direct_scf_tol %g, "
-                        "not %g (upload the basis with this cutoff, or run one mjk_get_jk_device build at it)",
-                        b->tile_cutoff, cut);
+                        "no-allocation build: the basis's quartet task lists are built for direct_scf_tol %g "
+                        "and tuning table %d, not %g and %d (upload the basis with this cutoff, or run one "
+                        "mjk_get_jk_device build)",
+                        b->tile_cutoff, b->tune_gen, cut, ctx->tune_gen);
         if ((rc = build_tiles(b, cut))) return rc;
     }
     const int n_j = with_j ? n_dm : 0;
@@ -1614,7 +1739,7 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
         // row kernel (one block per bra pair, K' rows in LDS) when the rows fit
         const size_t shmem = row_shmem(qc, n_j, n_k, b->maxld);
         int rr = -1;   // -1: no row kernel for this class (cooperative classes)
-        if (shmem <= ctx->row_lds && qc.nrt > 0) {
+        if (shmem <= ctx->row_lds && qc.nrt > 0 && qc.kernel_pref != 2) {
             a.rt = qc.rt;
             a.rr = qc.rr;
             a.rtp = qc.rtp;
@@ -1835,6 +1960,42 @@ int mjk_basis_set_class_filter(mjk_ctx* ctx, mjk_basis* b, int cls)
     return MJK_OK;
 }
 
+int mjk_ctx_set_tuning(mjk_ctx* ctx, int n, const int* keys, const double* scale)
+{
+    if (!ctx || n < 0 || (n > 0 && (!keys || !scale))) return fail(ctx, MJK_EINVAL, "bad arguments to mjk_ctx_set_tuning");
+    std::map<std::array<int, 9>, std::tuple<int, int, double>> t;
+    for (int i = 0; i < n; ++i) {
+        const int* r = keys + 11 * i;
+        for (int j = 0; j < 4; ++j)
+            if (r[j] < 0 || r[j] > LMAX || r[4 + j] < 1 || r[4 + j] > 2)
+                return fail(ctx, MJK_EINVAL, "tuning entry %d: bad class key", i);
+        if (r[9] < 0 || r[9] > 2 || r[10] < 0 || r[10] > 4096 || !(scale[i] >= 0 && scale[i] <= 1e6))
+            return fail(ctx, MJK_EINVAL, "tuning entry %d: bad setting", i);
+        t[{r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8]}] = std::make_tuple(r[9], r[10], scale[i]);
+    }
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    ctx->tune.swap(t);
+    ++ctx->tune_gen;   // every basis rebuilds its task lists at its next build
+    return MJK_OK;
+}
+
+int mjk_ctx_tuning_entries(mjk_ctx* ctx)
+{
+    if (!ctx) return MJK_EINVAL;
+    std::lock_guard<std::recursive_mutex> lock(ctx->mu);
+    return (int)ctx->tune.size();
+}
+
+int mjk_basis_class_key(mjk_ctx* ctx, const mjk_basis* b, int k, int* key9)
+{
+    if (!ctx || !b || b->ctx != ctx || k < 0 || k >= (int)b->qcls.size() || !key9)
+        return fail(ctx, MJK_EINVAL, "bad arguments to mjk_basis_class_key");
+    const QClass& q = b->qcls[k];
+    const int v[9] = {q.la, q.lb, q.lc, q.ld, q.ca, q.cb, q.cc, q.cd, q.bucket};
+    for (int i = 0; i < 9; ++i) key9[i] = v[i];
+    return MJK_OK;
+}
+
 int mjk_basis_nclasses(const mjk_basis* b)
 {
     return b ? (int)b->qcls.size() : MJK_EINVAL;
@@ -1858,7 +2019,7 @@ int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* b, int k, int n_dm, int hermi
     } else if (qc.mode == 2) {
         *kernel = 3;
     } else {
-        const bool row = row_shmem(qc, n_j, n_k, b->maxld) <= ctx->row_lds && qc.nrt > 0;
+        const bool row = row_shmem(qc, n_j, n_k, b->maxld) <= ctx->row_lds && qc.nrt > 0 && qc.kernel_pref != 2;
         *kernel = row ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
     }
     return MJK_OK;

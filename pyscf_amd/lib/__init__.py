@@ -15,6 +15,28 @@ _LIBDIR = os.path.dirname(os.path.abspath(__file__))
 _loaded = {}
 
 
+class mjk_rhf_result(ctypes.Structure):
+    _fields_ = [('e_tot', ctypes.c_double), ('gnorm', ctypes.c_double), ('cycles', ctypes.c_int),
+                ('converged', ctypes.c_int)]
+
+
+class mjk_request(ctypes.Structure):
+    _fields_ = [('kind', ctypes.c_int), ('atm', ctypes.POINTER(ctypes.c_int)), ('natm', ctypes.c_int),
+                ('bas', ctypes.POINTER(ctypes.c_int)), ('nbas', ctypes.c_int),
+                ('env', ctypes.POINTER(ctypes.c_double)), ('env_size', ctypes.c_int),
+                ('direct_scf_tol', ctypes.c_double), ('fp32_tol', ctypes.c_double),
+                ('dm', ctypes.POINTER(ctypes.c_double)), ('n_dm', ctypes.c_int), ('hermi', ctypes.c_int),
+                ('with_j', ctypes.c_int), ('with_k', ctypes.c_int), ('nelectron', ctypes.c_int),
+                ('max_cycle', ctypes.c_int), ('conv_tol', ctypes.c_double)]
+
+
+class mjk_request_info(ctypes.Structure):
+    _fields_ = [('state', ctypes.c_int), ('nao', ctypes.c_int), ('n_dm', ctypes.c_int), ('cycles', ctypes.c_int),
+                ('converged', ctypes.c_int), ('slot', ctypes.c_int), ('e_tot', ctypes.c_double),
+                ('t_submit', ctypes.c_double), ('t_start', ctypes.c_double), ('t_done', ctypes.c_double),
+                ('hbm_estimate', ctypes.c_size_t), ('hbm_peak', ctypes.c_size_t), ('error', ctypes.c_char * 256)]
+
+
 class mjk_stats(ctypes.Structure):
     _fields_ = [('ms_total', ctypes.c_double),
                 ('ms_quartets', ctypes.c_double),
@@ -60,6 +82,28 @@ def _declare(lib):
     lib.mjk_basis_fetch_jk.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_void_p]
     lib.mjk_basis_set_class_filter.argtypes = [c_void_p, c_void_p, c_int]
     lib.mjk_basis_nclasses.argtypes = [c_void_p]
+    lib.mjk_basis_class_key.argtypes = [c_void_p, c_void_p, c_int, c_void_p]
+    lib.mjk_ctx_set_tuning.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+    lib.mjk_ctx_tuning_entries.argtypes = [c_void_p]
+    lib.mjk_rhf_create.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, P(c_void_p)]
+    lib.mjk_rhf_run.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double, c_double,
+                                c_double, P(mjk_rhf_result)]
+    lib.mjk_rhf_fetch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.mjk_rhf_device_density.argtypes = [c_void_p]
+    lib.mjk_rhf_device_density.restype = c_void_p
+    lib.mjk_rhf_destroy.argtypes = [c_void_p]
+    lib.mjk_rhf_destroy.restype = None
+    lib.mjk_server_create.argtypes = [c_int, c_int, c_size_t, P(c_void_p)]
+    lib.mjk_submit.argtypes = [c_void_p, P(mjk_request), P(ctypes.c_longlong)]
+    lib.mjk_poll.argtypes = [c_void_p, ctypes.c_longlong, P(mjk_request_info)]
+    lib.mjk_wait_any.argtypes = [c_void_p, c_double, P(ctypes.c_longlong)]
+    lib.mjk_result.argtypes = [c_void_p, ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.mjk_server_hbm_held.argtypes = [c_void_p]
+    lib.mjk_server_hbm_held.restype = c_size_t
+    lib.mjk_server_error.argtypes = [c_void_p]
+    lib.mjk_server_error.restype = ctypes.c_char_p
+    lib.mjk_server_destroy.argtypes = [c_void_p]
+    lib.mjk_server_destroy.restype = None
     lib.mjk_basis_class.argtypes = [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
                                     P(ctypes.c_longlong), P(c_int)]
     lib.mjk_device_alloc.argtypes = [c_void_p, c_size_t, P(c_void_p)]

@@ -208,3 +208,57 @@ class RHF:
 
     def make_rdm1_final(self):
         return self.make_rdm1(self.mo_coeff, self.mo_occ)
+
+
+class DeviceRHF(RHF):
+    """RHF whose whole loop runs on the device (mjk_rhf_create/run/fetch):
+    the density, Fock matrix, DIIS history and MO coefficients stay in HBM,
+    the dense algebra runs on rocBLAS/rocSOLVER, J/K on the engine.  Same
+    inputs, attributes and convergence test as RHF; the host sees the DIIS
+    overlaps and three scalars per cycle.  The initial guess is the core
+    Hamiltonian, or dm0."""
+
+    def kernel(self, dm0=None):
+        mol = self.mol
+        if mol.nelectron % 2:
+            raise ValueError('RHF needs an even number of electrons')
+        h1e = numpy.ascontiguousarray(self.get_hcore())
+        s1e = numpy.ascontiguousarray(self.get_ovlp())
+        enuc = float(mol.energy_nuc())
+        conv_tol_grad = self.conv_tol_grad or numpy.sqrt(self.conv_tol)
+        if self._opt is None:
+            self._opt = {}
+        if self._opt.get(None) is None:
+            self._opt[None] = self.init_direct_scf(mol)
+        opt = self._opt[None]
+        eng, L = opt.engine, opt.engine.lib
+        nao = s1e.shape[0]
+        p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        h = ctypes.c_void_p()
+        eng.check(L.mjk_rhf_create(eng.handle, opt.basis.handle, p(s1e), p(h1e), mol.nelectron // 2, enuc,
+                                   self.diis_space, ctypes.byref(h)), 'mjk_rhf_create')
+        try:
+            if dm0 is not None:
+                dm0 = numpy.ascontiguousarray(dm0, dtype=numpy.double)
+            res = _lib.mjk_rhf_result()
+            t0 = time.perf_counter()
+            eng.check(L.mjk_rhf_run(eng.handle, h, p(dm0), self.max_cycle, self.conv_tol, conv_tol_grad,
+                                    self.direct_scf_tol, self.fp32_tol, self.level_shift, ctypes.byref(res)),
+                      'mjk_rhf_run')
+            self.seconds = time.perf_counter() - t0
+            dm = numpy.empty((nao, nao))
+            mo_energy = numpy.empty(nao)
+            mo_coeff = numpy.empty((nao, nao))
+            eng.check(L.mjk_rhf_fetch(eng.handle, h, p(dm), p(mo_energy), p(mo_coeff), None), 'mjk_rhf_fetch')
+        finally:
+            L.mjk_rhf_destroy(h)
+        self.e_tot = res.e_tot
+        self.converged = bool(res.converged)
+        self.cycles = res.cycles
+        self.gnorm = res.gnorm
+        self.mo_energy, self.mo_coeff = mo_energy, mo_coeff
+        self.mo_occ = self.get_occ(mo_energy)
+        self.dm = dm
+        return self.e_tot
+
+    scf = kernel

@@ -55,3 +55,23 @@ def test_rys_table_matches_boys_moments(oracle, n):
             assert abs(numpy.dot(w, x**m) - ref) <= 2e-14 * ref, (n, T, m)
         xo, wo = oracle.rys_roots(n, float(T))
         assert abs(x - xo).max() < 1e-13 and (abs(w - wo) / wo).max() < 1e-12
+
+
+def test_server_and_scf_bad_arguments():
+    l = lib.load_library()
+    h = ctypes.c_void_p()
+    t = ctypes.c_longlong()
+    assert l.mjk_server_create(0, 0, 0, ctypes.byref(h)) == -1
+    assert l.mjk_server_create(0, 65, 0, ctypes.byref(h)) == -1
+    assert l.mjk_submit(None, None, ctypes.byref(t)) == -1
+    assert l.mjk_poll(None, 1, None) == -1
+    assert l.mjk_wait_any(None, 0.0, ctypes.byref(t)) == -1
+    assert l.mjk_result(None, 1, None, None, None, None) == -1
+    assert l.mjk_server_hbm_held(None) == 0
+    l.mjk_server_destroy(None)
+    assert l.mjk_rhf_create(None, None, None, None, 1, 0.0, 8, ctypes.byref(h)) == -1
+    assert l.mjk_rhf_run(None, None, None, 10, 1e-9, 1e-5, 1e-13, 0.0, 0.0, None) == -1
+    assert l.mjk_rhf_fetch(None, None, None, None, None, None) == -1
+    assert l.mjk_rhf_device_density(None) is None
+    l.mjk_rhf_destroy(None)
+    assert l.mjk_ctx_tuning_entries(None) == -1
