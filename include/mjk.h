@@ -222,6 +222,35 @@ int mjk_rhf_fetch(mjk_ctx* ctx, mjk_scf* scf, double* dm, double* mo_energy, dou
 const double* mjk_rhf_device_density(const mjk_scf* scf);
 void mjk_rhf_destroy(mjk_scf* scf);
 
+/* Density-fitted J/K (pyscf/df/incore.py:129-214 cholesky_eri; pyscf/df/df_jk.py:278-418
+ * get_jk), Coulomb operator.
+ *   mjk_df_build: the orbital basis (atm/bas/env, shells up to f) and the
+ *     auxiliary basis (its own atm/bas/env, shells up to g, as auxmol holds
+ *     them).  Computes (P|Q) and (ij|P) on the device, then B = L^-1 (P|ij)
+ *     with (P|Q) = L L^T; when the factorisation fails, B = (V w^-1/2)^T (P|ij)
+ *     over the eigenpairs with w > lindep.  B stays in HBM (naux x nao^2, all
+ *     orbital pairs).  lindep < 0 keeps the raw (ij|P) (no fit; for tests).
+ *   mjk_df_naux: fitted functions (rows of B).
+ *   mjk_df_get_jk: n_dm densities (host, nao x nao each, any symmetry);
+ *     J = sum_P B_P tr(B_P D); K = sum_P B_P D B_P, or, when orbo != NULL
+ *     (n_dm x nao x nocc, D = X X^T per density, the reference's mo_coeff
+ *     path), K = sum_P (B_P X)(B_P X)^T.  vj/vk host, n_dm x nao x nao.
+ *   mjk_df_get_jk_device: the same with device arrays, asynchronous.
+ *   mjk_df_fetch: which 0 = the raw (P|Q) (naux_raw^2), 1 = B as the device
+ *     holds it, C order (nao, nao, naux): the reference's _cderi unpacked and
+ *     transposed. */
+typedef struct mjk_df mjk_df;
+int mjk_df_build(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas, const double* env,
+                 const int* aux_atm, int aux_natm, const int* aux_bas, int aux_nbas, const double* aux_env,
+                 double lindep, mjk_df** df);
+int mjk_df_naux(const mjk_df* df);
+int mjk_df_get_jk(mjk_ctx* ctx, mjk_df* df, const double* dms, int n_dm, const double* orbo, int nocc, int with_j,
+                  int with_k, double* vj, double* vk);
+int mjk_df_get_jk_device(mjk_ctx* ctx, mjk_df* df, const double* d_dms, int n_dm, const double* d_orbo, int nocc,
+                         int with_j, int with_k, double* d_vj, double* d_vk);
+int mjk_df_fetch(mjk_ctx* ctx, mjk_df* df, int which, double* out);
+void mjk_df_destroy(mjk_df* df);
+
 /* Request server (continuous batching on one device).  The reference serves
  * one caller at a time: each SCF.get_jk call crosses into CVHFnr_direct_drv
  * once (pyscf/scf/_vhf.py:540-639) and returns.  Here requests -- a J/K build

@@ -363,6 +363,8 @@ static int cart_index(int l, int lx, int ly)
         return idx + (l - lx - ly);
 }
 
+void orc_c2s_formula(int l, double *out);
+
 static void init_c2s(void)
 {
         memset(c2s_tab, 0, sizeof(c2s_tab));
@@ -402,6 +404,58 @@ static void init_c2s(void)
         c2s_tab[3][5][cart_index(3, 0, 2)] = -g2;
         c2s_tab[3][6][cart_index(3, 3, 0)] = f3;             /* x(x^2-3y^2) */
         c2s_tab[3][6][cart_index(3, 1, 2)] = -3 * f3;
+        /* l = 4 (auxiliary g functions) from the general formula */
+        {
+                double g4[9 * 15];
+                int m, c;
+                orc_c2s_formula(4, g4);
+                for (m = 0; m < 9; m++)
+                        for (c = 0; c < 15; c++) c2s_tab[4][m][c] = g4[m * 15 + c];
+        }
+}
+
+/* Real solid harmonics r^l Y_lm in Cartesian monomials for any l, from the
+ * closed formula of Helgaker, Jorgensen, Olsen, Molecular Electronic-Structure
+ * Theory, eqs. 6.4.47-6.4.50: S_lm = N_lm sum_{t,u,v} C_tuv x^(2t+|m|-2(u+v))
+ * y^(2(u+v)) z^(l-2t-|m|), with v = v_m, v_m + 1, ... (v_m = 0 for m >= 0,
+ * 1/2 for m < 0), C_tuv = (-1)^(t+v-v_m) 4^-t binom(l,t) binom(l-t,|m|+t)
+ * binom(t,u) binom(|m|,2v), N_lm = sqrt(2 (l+|m|)! (l-|m|)! / 2^delta(m,0))
+ * / (2^|m| l!), times sqrt((2l+1)/(4 pi)) for unit-sphere normalisation.
+ * Rows m = -l..l (the AO order for l >= 2). */
+static double binom(int n, int k)
+{
+        if (k < 0 || k > n) return 0.;
+        double r = 1.;
+        int i;
+        for (i = 1; i <= k; i++) r = r * (n - k + i) / i;
+        return r;
+}
+
+static double fact(int n)
+{
+        double r = 1.;
+        int i;
+        for (i = 2; i <= n; i++) r *= i;
+        return r;
+}
+
+void orc_c2s_formula(int l, double *out)
+{
+        int nc = ncart(l), m, t, u, v2;
+        memset(out, 0, sizeof(double) * (2 * l + 1) * nc);
+        for (m = -l; m <= l; m++) {
+                int am = m < 0 ? -m : m, v0 = m < 0 ? 1 : 0;
+                double N = sqrt(2. * fact(l + am) * fact(l - am) / (m == 0 ? 2. : 1.)) / (pow(2., am) * fact(l))
+                           * sqrt((2. * l + 1.) / (4. * M_PI));
+                for (t = 0; t <= (l - am) / 2; t++)
+                for (u = 0; u <= t; u++)
+                for (v2 = v0; v2 <= am; v2 += 2) {
+                        int sgn = ((t + (v2 - v0) / 2) % 2) ? -1 : 1;
+                        double C = sgn * pow(0.25, t) * binom(l, t) * binom(l - t, am + t) * binom(t, u) * binom(am, v2);
+                        int px = 2 * t + am - 2 * u - v2, py = 2 * u + v2;
+                        out[(m + l) * nc + cart_index(l, px, py)] += N * C;
+                }
+        }
 }
 
 static int c2s_ready = 0;
@@ -1050,4 +1104,116 @@ void orc_set_threads(int n)
 #ifdef _OPENMP
         if (n > 0) omp_set_num_threads(n);
 #endif
+}
+
+/* ---------------- three- and two-centre Coulomb integrals ---------------- */
+/* int3c2e / int2c2e of density fitting (pyscf/df/incore.py:34-124): the
+ * Coulomb integral (ij|P) of two orbital functions and one auxiliary
+ * function, and (P|Q).  Both are four-centre integrals with a unit s
+ * function (exponent 0, coefficient 1) in the empty slots, whose factor is
+ * exactly 1; only the real indices are transformed to spherical functions.
+ * bas holds the orbital shells [0, nbas), the auxiliary shells [nbas, nbas +
+ * naux_sh) and the unit shell at nbas + naux_sh. */
+static void sph_rows(double *out, const double *cart, int l, int nctr, size_t pre, size_t post)
+{
+        /* out[x][ctr*ns+m][y] = sum_c c2s[l][m][c] cart[x][ctr*nc+c][y] */
+        int nc = ncart(l), ns = 2 * l + 1, ctr, m, c;
+        size_t x, y;
+        for (x = 0; x < pre; x++)
+        for (ctr = 0; ctr < nctr; ctr++)
+        for (m = 0; m < ns; m++)
+        for (y = 0; y < post; y++) {
+                double v = 0;
+                for (c = 0; c < nc; c++) v += c2s_tab[l][m][c] * cart[(x * nctr * nc + ctr * nc + c) * post + y];
+                out[(x * nctr * ns + ctr * ns + m) * post + y] = v;
+        }
+}
+
+void orc_int3c2e(double *out, const int *ao_loc, int nbas, const int *aux_loc, int naux_sh,
+                 const int *atm, int natm, const int *bas, const double *env)
+{
+        ensure_init();
+        size_t nao = ao_loc[nbas], naux = aux_loc[naux_sh];
+#pragma omp parallel
+{
+        Shell si, sj, sk, s0;
+        get_shell(&s0, nbas + naux_sh, atm, bas, env);
+        size_t maxc = 0;
+        int a;
+        for (a = 0; a < nbas + naux_sh; a++) {
+                const int *b = bas + a * BAS_SLOTS;
+                size_t n = (size_t)ncart(b[ANG_OF]) * b[NCTR_OF];
+                if (n > maxc) maxc = n;
+        }
+        double *c1 = malloc(sizeof(double) * maxc * maxc * maxc);
+        double *c2 = malloc(sizeof(double) * maxc * maxc * maxc);
+        int ish, jsh, ksh;
+#pragma omp for schedule(dynamic) collapse(2)
+        for (ish = 0; ish < nbas; ish++)
+        for (jsh = 0; jsh < nbas; jsh++) {
+                if (jsh > ish) continue;
+                get_shell(&si, ish, atm, bas, env);
+                get_shell(&sj, jsh, atm, bas, env);
+                for (ksh = 0; ksh < naux_sh; ksh++) {
+                        get_shell(&sk, nbas + ksh, atm, bas, env);
+                        int Nj = ncart(sj.l) * sj.nctr, Nk = ncart(sk.l) * sk.nctr;
+                        int Si = (2 * si.l + 1) * si.nctr, Sj = (2 * sj.l + 1) * sj.nctr, Sk = (2 * sk.l + 1) * sk.nctr;
+                        eri_cart(c1, &si, &sj, &sk, &s0, 0.);
+                        sph_rows(c2, c1, si.l, si.nctr, 1, (size_t)Nj * Nk);
+                        sph_rows(c1, c2, sj.l, sj.nctr, Si, Nk);
+                        sph_rows(c2, c1, sk.l, sk.nctr, (size_t)Si * Sj, 1);
+                        int i, j, k;
+                        for (i = 0; i < Si; i++)
+                        for (j = 0; j < Sj; j++)
+                        for (k = 0; k < Sk; k++) {
+                                double v = c2[((size_t)i * Sj + j) * Sk + k];
+                                size_t I = ao_loc[ish] + i, J = ao_loc[jsh] + j, K = aux_loc[ksh] + k;
+                                out[(I * nao + J) * naux + K] = v;
+                                out[(J * nao + I) * naux + K] = v;
+                        }
+                }
+        }
+        free(c1);
+        free(c2);
+}
+}
+
+void orc_int2c2e(double *out, const int *aux_loc, int nbas, int naux_sh,
+                 const int *atm, int natm, const int *bas, const double *env)
+{
+        ensure_init();
+        size_t naux = aux_loc[naux_sh];
+#pragma omp parallel
+{
+        Shell sp, sq, s0;
+        get_shell(&s0, nbas + naux_sh, atm, bas, env);
+        size_t maxc = 0;
+        int a;
+        for (a = nbas; a < nbas + naux_sh; a++) {
+                const int *b = bas + a * BAS_SLOTS;
+                size_t n = (size_t)ncart(b[ANG_OF]) * b[NCTR_OF];
+                if (n > maxc) maxc = n;
+        }
+        double *c1 = malloc(sizeof(double) * maxc * maxc);
+        double *c2 = malloc(sizeof(double) * maxc * maxc);
+        int psh, qsh;
+#pragma omp for schedule(dynamic)
+        for (psh = 0; psh < naux_sh; psh++) {
+                get_shell(&sp, nbas + psh, atm, bas, env);
+                for (qsh = 0; qsh < naux_sh; qsh++) {
+                        get_shell(&sq, nbas + qsh, atm, bas, env);
+                        int Nq = ncart(sq.l) * sq.nctr;
+                        int Sp = (2 * sp.l + 1) * sp.nctr, Sq = (2 * sq.l + 1) * sq.nctr;
+                        eri_cart(c1, &sp, &s0, &sq, &s0, 0.);
+                        sph_rows(c2, c1, sp.l, sp.nctr, 1, Nq);
+                        sph_rows(c1, c2, sq.l, sq.nctr, Sp, 1);
+                        int p, q;
+                        for (p = 0; p < Sp; p++)
+                        for (q = 0; q < Sq; q++)
+                                out[(size_t)(aux_loc[psh] + p) * naux + aux_loc[qsh] + q] = c1[(size_t)p * Sq + q];
+                }
+        }
+        free(c1);
+        free(c2);
+}
 }

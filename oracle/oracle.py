@@ -294,3 +294,88 @@ def rhf(mol, conv_tol=1e-10, max_cycle=100, direct_scf_tol=1e-13, get_jk_fn=None
     vhf = vj - .5 * vk
     e_tot = numpy.einsum('ij,ji->', dm, H) + .5 * numpy.einsum('ij,ji->', dm, vhf) + enuc
     return e_tot, dm, e
+
+
+# ---------------- density fitting (pyscf/df/incore.py, pyscf/df/df_jk.py) ----------------
+
+def _df_arrays(mol, auxmol):
+    """atm/bas/env with the orbital shells, the auxiliary shells and a unit s
+    shell (exponent 0, coefficient 1) appended, as gto.conc_env joins them
+    (pyscf/df/incore.py:80-84)."""
+    atm, bas, env, ao_loc = _mol_arrays(mol)
+    aatm, abas, aenv, aux_loc = _mol_arrays(auxmol)
+    off = len(env)
+    abas = abas.copy()
+    abas[:, 5] += off
+    abas[:, 6] += off
+    aatm = aatm.copy()
+    aatm[:, 1] += off
+    aatm[:, 3] += off
+    unit = numpy.array([[0, 0, 1, 1, 0, off + len(aenv), off + len(aenv) + 1, 0]], dtype=numpy.int32)
+    env2 = numpy.concatenate([env, aenv, [0.0, 1.0]])
+    abas[:, 0] += len(atm)
+    bas2 = numpy.ascontiguousarray(numpy.vstack([bas, abas, unit]), dtype=numpy.int32)
+    atm2 = numpy.ascontiguousarray(numpy.vstack([atm, aatm]), dtype=numpy.int32)
+    return atm2, bas2, env2, ao_loc, aux_loc, len(bas), len(abas)
+
+
+def int3c2e(mol, auxmol):
+    """(ij|P), shape (nao, nao, naux) (aux_e2 with aosym='s1',
+    pyscf/df/incore.py:34-84)."""
+    atm, bas, env, ao_loc, aux_loc, nbas, naux_sh = _df_arrays(mol, auxmol)
+    nao, naux = int(ao_loc[-1]), int(aux_loc[-1])
+    out = numpy.zeros((nao, nao, naux))
+    lib().orc_int3c2e(_p(out), _p(ao_loc), ctypes.c_int(nbas), _p(aux_loc), ctypes.c_int(naux_sh),
+                      _p(atm), ctypes.c_int(len(atm)), _p(bas), _p(env))
+    return out
+
+
+def int2c2e(mol, auxmol):
+    """(P|Q), shape (naux, naux) (fill_2c2e, pyscf/df/incore.py:110-124)."""
+    atm, bas, env, ao_loc, aux_loc, nbas, naux_sh = _df_arrays(mol, auxmol)
+    naux = int(aux_loc[-1])
+    out = numpy.zeros((naux, naux))
+    lib().orc_int2c2e(_p(out), _p(aux_loc), ctypes.c_int(nbas), ctypes.c_int(naux_sh),
+                      _p(atm), ctypes.c_int(len(atm)), _p(bas), _p(env))
+    return out
+
+
+def c2s_formula(l):
+    nc = (l + 1) * (l + 2) // 2
+    out = numpy.zeros((2 * l + 1, nc))
+    lib().orc_c2s_formula(ctypes.c_int(l), _p(out))
+    return out
+
+
+def cholesky_eri(mol, auxmol, lindep=1e-7):
+    """cderi (naux, nao, nao) = L^-1 (P|ij) with (P|Q) = L L^T; when the
+    Cholesky factorisation fails, the eigenvectors of (P|Q) above lindep
+    scaled by w^-1/2 (pyscf/df/incore.py:129-214)."""
+    import scipy.linalg
+    j2c = int2c2e(mol, auxmol)
+    j3c = int3c2e(mol, auxmol)
+    nao, naux = j3c.shape[0], j3c.shape[2]
+    try:
+        low = scipy.linalg.cholesky(j2c, lower=True)
+        cderi = scipy.linalg.solve_triangular(low, j3c.reshape(-1, naux).T, lower=True)
+    except scipy.linalg.LinAlgError:
+        w, v = scipy.linalg.eigh(j2c)
+        keep = w > lindep
+        cderi = (v[:, keep] / numpy.sqrt(w[keep])).T @ j3c.reshape(-1, naux).T
+    return cderi.reshape(-1, nao, nao)
+
+
+def df_get_jk(cderi, dms, hermi=0, with_j=True, with_k=True):
+    """DF J and K of real densities (pyscf/df/df_jk.py:278-418):
+    J = sum_P B_P tr(B_P D), K = sum_P B_P D B_P."""
+    dms = numpy.asarray(dms, dtype=numpy.double)
+    shape = dms.shape
+    nao = shape[-1]
+    dms = dms.reshape(-1, nao, nao)
+    vj = vk = None
+    if with_j:
+        rho = numpy.einsum('pij,nji->np', cderi, dms)
+        vj = numpy.einsum('np,pij->nij', rho, cderi).reshape(shape)
+    if with_k:
+        vk = numpy.array([numpy.einsum('pik,kl,plj->ij', cderi, d, cderi, optimize=True) for d in dms]).reshape(shape)
+    return vj, vk

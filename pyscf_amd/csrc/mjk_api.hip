@@ -1287,6 +1287,49 @@ int ensure(mjk_ctx* ctx, double** p, long long& cap, long long need)
 
 namespace {
 
+// Real solid harmonics r^l Y_lm (unit-sphere normalised) as rows m = -l..l over
+// the Cartesian monomials of angular momentum l (order lx = l..0, ly =
+// l-lx..0), from the closed formula (Helgaker, Jorgensen, Olsen, Molecular
+// Electronic-Structure Theory, eqs. 6.4.47-6.4.50):
+//   S_lm = N_lm sum_{t, u, v} (-1)^(t+v-v_m) 4^-t C(l,t) C(l-t,|m|+t) C(t,u) C(|m|,2v)
+//          x^(2t+|m|-2(u+v)) y^(2(u+v)) z^(l-2t-|m|),
+//   v = v_m, v_m + 1, ... <= |m|/2, v_m = 0 (m >= 0) or 1/2 (m < 0),
+//   N_lm = sqrt(2 (l+|m|)! (l-|m|)! / 2^delta(m,0)) / (2^|m| l!),
+// times sqrt((2l+1)/(4 pi)).  Used for l = 4 (auxiliary g functions); the
+// l <= 3 tables of c2s.h agree with it.
+void solid_harmonic_rows(int l, double* out)
+{
+    auto binom = [](int n, int k) {
+        if (k < 0 || k > n) return 0.0;
+        double r = 1.0;
+        for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+        return r;
+    };
+    auto fact = [](int n) {
+        double r = 1.0;
+        for (int i = 2; i <= n; ++i) r *= i;
+        return r;
+    };
+    const int nc = (l + 1) * (l + 2) / 2;
+    std::fill(out, out + (2 * l + 1) * nc, 0.0);
+    for (int m = -l; m <= l; ++m) {
+        const int am = std::abs(m), v0 = m < 0 ? 1 : 0;
+        const double N = std::sqrt(2.0 * fact(l + am) * fact(l - am) / (m == 0 ? 2.0 : 1.0)) /
+                         (std::pow(2.0, am) * fact(l)) * std::sqrt((2.0 * l + 1.0) / (4.0 * M_PI));
+        for (int t = 0; t <= (l - am) / 2; ++t)
+            for (int u = 0; u <= t; ++u)
+                for (int v2 = v0; v2 <= am; v2 += 2) {
+                    const double sgn = ((t + (v2 - v0) / 2) % 2) ? -1.0 : 1.0;
+                    const double C = sgn * std::pow(0.25, t) * binom(l, t) * binom(l - t, am + t) * binom(t, u) *
+                                     binom(am, v2);
+                    const int px = 2 * t + am - 2 * u - v2, py = 2 * u + v2;
+                    const int r = l - px;   // = py + pz
+                    const int pz = l - px - py;
+                    out[(m + l) * nc + r * (r + 1) / 2 + pz] += N * C;
+                }
+    }
+}
+
 // The tuning table the context starts with: the file MJK_TUNING names, else
 // tuning/mi355x.txt beside the library's lib/ folder; none when MJK_TUNE=0.
 // Lines: la lb lc ld ca cb cc cd bucket kernel rprim scale [anything], '#'
@@ -1340,6 +1383,7 @@ size_t ctx_budget(const mjk_ctx* ctx) { return ctx->budget; }
 std::recursive_mutex& ctx_mutex(mjk_ctx* ctx) { return ctx->mu; }
 std::shared_mutex& capture_mutex() { return g_capture; }
 void* ctx_blas(mjk_ctx* ctx) { return ctx->blas; }
+const double* ctx_rys(const mjk_ctx* ctx) { return ctx->rys; }
 void ctx_set_blas(mjk_ctx* ctx, void* handle, void (*destroy)(void*))
 {
     ctx->blas = handle;
@@ -2153,7 +2197,11 @@ int mjk_synchronize(mjk_ctx* ctx)
 
 int mjk_c2s(int l, double* out)
 {
-    if (l < 0 || l > C2S_MAXL || !out) return MJK_EINVAL;
+    if (l < 0 || l > 4 || !out) return MJK_EINVAL;
+    if (l > C2S_MAXL) {
+        solid_harmonic_rows(l, out);
+        return MJK_OK;
+    }
     // l <= 1: identity blocks here, the angular factors sit in the contraction coefficients
     const double f = l == 0 ? C2S_S0 : l == 1 ? C2S_P1 : 1.0;
     for (int m = 0; m < nsph(l); ++m)

@@ -27,5 +27,9 @@ std::shared_mutex& capture_mutex();
 // under capture_mutex() and destroyed with the context by `destroy`.
 void* ctx_blas(mjk_ctx* ctx);
 void ctx_set_blas(mjk_ctx* ctx, void* handle, void (*destroy)(void*));
+// the context's rocBLAS handle (a rocblas_handle), made on first use; null on failure
+void* ctx_rocblas(mjk_ctx* ctx);
+// Rys root table of the context (device)
+const double* ctx_rys(const mjk_ctx* ctx);
 
 }  // namespace mjk_internal

@@ -232,12 +232,7 @@ void diis_solve(int m, const std::vector<double>& Bm, std::vector<double>& c)
 // stream capture: both run with no capture in flight.
 rocblas_handle blas_of(mjk_ctx* ctx)
 {
-    if (void* h = mjk_internal::ctx_blas(ctx)) return (rocblas_handle)h;
-    std::unique_lock<std::shared_mutex> lk(mjk_internal::capture_mutex());
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-    mjk_internal::ctx_set_blas(ctx, h, [](void* p) { (void)rocblas_destroy_handle((rocblas_handle)p); });
-    return h;
+    return (rocblas_handle)mjk_internal::ctx_rocblas(ctx);
 }
 
 int alloc(mjk_scf* s, double** p, size_t elems)
@@ -578,6 +573,20 @@ int run(mjk_scf* s, const double* dm0, int max_cycle, double conv_tol, double co
 }
 
 }  // namespace
+
+namespace mjk_internal {
+
+void* ctx_rocblas(mjk_ctx* ctx)
+{
+    if (void* h = ctx_blas(ctx)) return h;
+    std::unique_lock<std::shared_mutex> lk(capture_mutex());
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+    ctx_set_blas(ctx, h, [](void* p) { (void)rocblas_destroy_handle((rocblas_handle)p); });
+    return h;
+}
+
+}  // namespace mjk_internal
 
 extern "C" {
 

@@ -7,7 +7,9 @@ coefficients are all zero are dropped, and contractions are kept as
 written (no contraction optimisation, pyscf/gto/basis/__init__.py:475).
 
 Only the elements H, He, C, N, O, F, Ne of cc-pVDZ, def2-SVP, def2-TZVP
-and 6-31G are shipped (data files beside this module).
+and 6-31G, and of the fitting sets cc-pVDZ-JKFIT, def2-universal-JFIT
+('weigend') and def2-universal-JKFIT are shipped (data files beside this
+module).
 """
 
 import os
@@ -21,6 +23,14 @@ ALIAS = {
     'def2svp': 'def2-svp.dat',
     'def2tzvp': 'def2-tzvp.dat',
     '631g': '6-31g.dat',
+    # auxiliary (density fitting) sets, names of pyscf/gto/basis/__init__.py:67-197
+    'ccpvdzjkfit': 'cc-pvdz-jkfit.dat',
+    'def2universaljfit': 'def2-universal-jfit.dat',
+    'weigend': 'def2-universal-jfit.dat',
+    'weigendcfit': 'def2-universal-jfit.dat',
+    'def2universaljkfit': 'def2-universal-jkfit.dat',
+    'def2svpjkfit': 'def2-universal-jkfit.dat',
+    'def2tzvpjkfit': 'def2-universal-jkfit.dat',
 }
 
 

@@ -93,6 +93,15 @@ def _declare(lib):
     lib.mjk_rhf_device_density.restype = c_void_p
     lib.mjk_rhf_destroy.argtypes = [c_void_p]
     lib.mjk_rhf_destroy.restype = None
+    lib.mjk_df_build.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                 c_int, c_void_p, c_double, P(c_void_p)]
+    lib.mjk_df_naux.argtypes = [c_void_p]
+    lib.mjk_df_get_jk.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_void_p,
+                                  c_void_p]
+    lib.mjk_df_get_jk_device.argtypes = lib.mjk_df_get_jk.argtypes
+    lib.mjk_df_fetch.argtypes = [c_void_p, c_void_p, c_int, c_void_p]
+    lib.mjk_df_destroy.argtypes = [c_void_p]
+    lib.mjk_df_destroy.restype = None
     lib.mjk_server_create.argtypes = [c_int, c_int, c_size_t, P(c_void_p)]
     lib.mjk_submit.argtypes = [c_void_p, P(mjk_request), P(ctypes.c_longlong)]
     lib.mjk_poll.argtypes = [c_void_p, ctypes.c_longlong, P(mjk_request_info)]
@@ -164,3 +173,20 @@ def hermi_triu(mat, hermi=1, inplace=True):
     else:
         raise ValueError('hermi must be 1 or 2')
     return mat
+
+
+class _TaggedArray(numpy.ndarray):
+    """ndarray carrying extra attributes (mo_coeff, mo_occ) through views."""
+
+    def __array_finalize__(self, obj):
+        if obj is not None:
+            self.__dict__.update(getattr(obj, '__dict__', {}))
+
+
+def tag_array(a, **kwargs):
+    """a with attributes attached, as pyscf.lib.tag_array
+    (pyscf/lib/numpy_helper.py): the density-fitted K builds use a density's
+    mo_coeff/mo_occ when they are there."""
+    t = numpy.asarray(a).view(_TaggedArray)
+    t.__dict__.update(kwargs)
+    return t

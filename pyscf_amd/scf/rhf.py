@@ -142,6 +142,11 @@ class RHF:
         e, c = numpy.linalg.eigh(x.T @ f @ x)
         return e, x @ c
 
+    def density_fit(self, auxbasis=None, with_df=None):
+        """This SCF with density-fitted J/K (pyscf/df/df_jk.py:32-125)."""
+        from pyscf_amd.df import df_jk
+        return df_jk.density_fit(self, auxbasis, with_df)
+
     def get_occ(self, mo_energy):
         nocc = self.mol.nelectron // 2
         occ = numpy.zeros_like(mo_energy)
