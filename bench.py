@@ -27,7 +27,7 @@ The JSON line (stdout) carries:
 
 Other modes: --workload c1/c2 (single molecules), c4 (256 chains batched,
 active-molecule builds/s against the same molecules' own RHF runs), c5
-(serving stream); --pmc-run DM.npy (two builds of a saved density and a
+(serving stream), c3df (density-fitted J/K of C30H62 with cc-pVDZ-JKFIT); --pmc-run DM.npy (two builds of a saved density and a
 64 MiB calibration copy, for rocprofv3 --pmc runs); --pmc-summary FETCH_DIR
 WRITE_DIR (per-kernel HBM bytes per launch from those runs' CSVs).
 """
@@ -94,7 +94,7 @@ def parse():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
     ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--workload', default='c3', choices=['c1', 'c2', 'c3', 'c4', 'c5'])
+    ap.add_argument('--workload', default='c3', choices=['c1', 'c2', 'c3', 'c4', 'c5', 'c3df'])
     ap.add_argument('--nmol', type=int, default=256, help='c4: molecules in the batch')
     ap.add_argument('--nreq', type=int, default=48, help='c5: requests in the stream')
     ap.add_argument('--rate', type=float, default=20.0, help='c5: Poisson arrival rate (1/s)')
@@ -573,6 +573,101 @@ def pmc_summary(args):
                       'kernels': kernels}, indent=1))
 
 
+def bench_df(args):
+    """Density-fitted J/K of C30H62 cc-pVDZ with cc-pVDZ-JKFIT (pyscf/df/df_jk.py:278-418):
+    the fitting tensor built once on the device, then timed J/K builds of the
+    converged DF-RHF density from its occupied orbitals (the SCF path) and of
+    the density itself (the general path), inputs and outputs resident in HBM."""
+    from pyscf_amd import df
+    from pyscf_amd.scf import _vhf, rhf
+    mol, desc = workload('c3')
+    nao = mol.nao_nr()
+    eng = _vhf.get_engine()
+    L = eng.lib
+    progress('df: fitting tensor')
+    t0 = time.perf_counter()
+    d = df.DF(mol).build()
+    t_fit = time.perf_counter() - t0
+    naux = d.get_naoaux()
+    progress('df: %d AOs, %d fitting functions, tensor %.1f GB in %.2f s' % (nao, naux, 8e-9 * naux * nao * nao, t_fit))
+    progress('df: DF-RHF')
+    mf = rhf.RHF(mol, direct_scf_tol=TOL).density_fit(with_df=d)
+    mf.conv_tol = 1e-10
+    t0 = time.perf_counter()
+    e_df = mf.kernel()
+    t_scf = time.perf_counter() - t0
+    progress('df: exact RHF for the fitting error')
+    mfx = rhf.RHF(mol, direct_scf_tol=TOL)
+    mfx.conv_tol = 1e-10
+    e_exact = mfx.kernel()
+    occ = mf.mo_occ > 0
+    orbo = numpy.ascontiguousarray(mf.mo_coeff[:, occ] * numpy.sqrt(mf.mo_occ[occ]))
+    dm = numpy.ascontiguousarray(orbo @ orbo.T)
+    nocc = orbo.shape[1]
+    # device copies of the inputs, device outputs
+    bufs = {}
+    for name, n in (('dm', nao * nao), ('orbo', nao * nocc), ('vj', nao * nao), ('vk', nao * nao)):
+        ptr = ctypes.c_void_p()
+        eng.check(L.mjk_device_alloc(eng.handle, 8 * n, ctypes.byref(ptr)), 'mjk_device_alloc')
+        bufs[name] = ptr
+    for name, a in (('dm', dm), ('orbo', orbo)):
+        eng.check(L.mjk_memcpy_h2d(eng.handle, bufs[name], a.ctypes.data_as(ctypes.c_void_p), a.nbytes), 'h2d')
+    eng.check(L.mjk_synchronize(eng.handle), 'sync')
+
+    def timed(orbital_path, steps, warmup):
+        po = bufs['orbo'] if orbital_path else None
+        for _ in range(warmup):
+            eng.check(L.mjk_df_get_jk_device(eng.handle, d.handle, bufs['dm'], 1, po, nocc, 1, 1, bufs['vj'],
+                                             bufs['vk']), 'mjk_df_get_jk_device')
+        eng.check(L.mjk_synchronize(eng.handle), 'sync')
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            eng.check(L.mjk_df_get_jk_device(eng.handle, d.handle, bufs['dm'], 1, po, nocc, 1, 1, bufs['vj'],
+                                             bufs['vk']), 'mjk_df_get_jk_device')
+            eng.check(L.mjk_synchronize(eng.handle), 'sync')
+            lat.append(time.perf_counter() - t1)
+        return (time.perf_counter() - t0) / steps, numpy.array(lat)
+
+    progress('df: %d timed builds from occupied orbitals' % args.steps)
+    t_mo, lat_mo = timed(True, args.steps, args.warmup)
+    nsteps_gen = max(3, args.steps // 4)
+    progress('df: %d timed builds of the density' % nsteps_gen)
+    t_gen, _ = timed(False, nsteps_gen, 1)
+    vj, vk = numpy.empty((nao, nao)), numpy.empty((nao, nao))
+    eng.check(L.mjk_memcpy_d2h(eng.handle, vj.ctypes.data_as(ctypes.c_void_p), bufs['vj'], vj.nbytes), 'd2h')
+    eng.check(L.mjk_memcpy_d2h(eng.handle, vk.ctypes.data_as(ctypes.c_void_p), bufs['vk'], vk.nbytes), 'd2h')
+    vj2, vk2 = df.get_jk(d, mf.make_rdm1(mf.mo_coeff, mf.mo_occ), 1)
+    for ptr in bufs.values():
+        L.mjk_device_free(eng.handle, ptr)
+    nn = nao * nao
+    flops_j = 2 * 2.0 * naux * nn
+    flops_mo = flops_j + 2.0 * naux * nao * nocc * nao + 2.0 * nn * naux * nocc
+    flops_gen = flops_j + 2 * 2.0 * naux * nao * nn
+    out = {
+        'metric': 'DF J/K builds/s (fitted three-centre tensor resident in HBM)', 'value': 1.0 / t_mo,
+        'unit': 'JK builds/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1e3 * t_mo, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic geometry; converged DF-RHF density from this engine',
+        'config': {'workload': desc + ', cc-pVDZ-JKFIT (%d fitting functions), J and K of the converged '
+                   'DF-RHF density from its %d occupied orbitals' % (naux, nocc)},
+        'p50_ms': float(numpy.percentile(lat_mo, 50) * 1e3), 'p99_ms': float(numpy.percentile(lat_mo, 99) * 1e3),
+        'general_density_path': {'ms_per_build': 1e3 * t_gen, 'builds_per_s': 1.0 / t_gen,
+                                 'tflops': flops_gen / t_gen * 1e-12},
+        'fit': {'seconds': t_fit, 'naux': naux, 'tensor_gb': 8e-9 * naux * nn},
+        'scf': {'e_df': e_df, 'e_exact': e_exact, 'fitting_error_eh': e_df - e_exact, 'cycles': mf.cycles,
+                'seconds': t_scf},
+        'paths_agree': {'max_dJ': float(abs(vj - vj2).max()), 'max_dK': float(abs(vk - vk2).max())},
+        'roofline': {'bound': 'mfma', 'achieved': flops_mo / t_mo * 1e-12, 'peak': FP64_PEAK * 1e-12,
+                     'unit': 'TFLOP/s', 'frac': flops_mo / t_mo / FP64_PEAK, 'traffic': None,
+                     'note': 'whole build (rocBLAS FP64 GEMMs: J two passes over the tensor, K a strided-batched '
+                             'GEMM and one GEMM), algorithmic FLOPs over wall time per synchronised build'},
+        'cpu_baseline': None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -589,6 +684,8 @@ def main():
         return bench_batched(args)
     if args.workload == 'c5':
         return bench_serving(args)
+    if args.workload == 'c3df':
+        return bench_df(args)
     return main_single(args)
 
 

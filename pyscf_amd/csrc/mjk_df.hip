@@ -11,7 +11,8 @@
 //   B stays in HBM as a column-major (naux x nao^2) matrix: for every orbital
 //   pair (i, j), both triangles, the naux fitted values are contiguous.
 // Builds (per call):
-//   * J: rho = B vec(D) and vec(J) = B^T rho (two GEMMs over all densities);
+//   * J: rho = B vec(D) and vec(J) = B^T rho, two bandwidth-bound passes over
+//     the lower-triangle rows of B (k_df_rho, k_df_j);
 //   * K of a density: Y_i = B_i D^T for every i (one strided-batched GEMM,
 //     B_i = the naux x nao block of row i), then K = B^T Y as one GEMM over
 //     (P, l): K_ik = sum_Pl B_k[P,l] Y_i[P,l] = sum_P (B_P D B_P)_ik;
@@ -125,6 +126,85 @@ __global__ void k_df_auxsph(const double* __restrict__ S, long long lds, double*
     double v = 0.0;
     for (int c = 0; c < nc; ++c) v += tr[c] * s[(long long)c * ngrp[l]];
     B[row * naux + P] = v;
+}
+
+// J of one density from the fitted tensor, two passes over the lower-triangle
+// rows (i >= j) of B, which is symmetric in (i, j): half the tensor's bytes
+// per pass.  These are bandwidth-bound matrix-vector products; as GEMMs with
+// one column they ran at a few percent of HBM bandwidth.
+//   rho[P] = sum_{i>=j} w_ij B[ij][P],  w_ij = D_ij + D_ji (i > j), D_ii (i = j)
+// Grid: x = chunks of DF_ROWS lower-triangle rows, y = column tiles of
+// DF_COLS columns; a lane keeps DF_COLS / 256 column sums over the chunk's
+// rows, then adds them into rho (FP64 atomics; rho is zeroed first).
+constexpr int DF_ROWS = 256;
+constexpr int DF_CPT = 8;                 // columns per lane
+constexpr int DF_COLS = 256 * DF_CPT;     // columns per block
+
+__device__ __forceinline__ void tri_index(long long t, int& i, int& j)
+{
+    int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((long long)(r + 1) * (r + 2) / 2 <= t) ++r;
+    while ((long long)r * (r + 1) / 2 > t) --r;
+    i = r;
+    j = (int)(t - (long long)r * (r + 1) / 2);
+}
+
+__global__ void __launch_bounds__(256) k_df_rho(const double* __restrict__ B, int nao, int naux,
+                                                const double* __restrict__ D, double* __restrict__ rho)
+{
+    const long long ntri = (long long)nao * (nao + 1) / 2;
+    const long long t0 = (long long)blockIdx.x * DF_ROWS;
+    const long long t1 = t0 + DF_ROWS < ntri ? t0 + DF_ROWS : ntri;
+    const int c0 = blockIdx.y * DF_COLS + threadIdx.x;
+    double acc[DF_CPT];
+#pragma unroll
+    for (int k = 0; k < DF_CPT; ++k) acc[k] = 0.0;
+    int i, j;
+    tri_index(t0, i, j);
+    for (long long t = t0; t < t1; ++t) {
+        const double w = i == j ? D[(long long)i * nao + i] : D[(long long)i * nao + j] + D[(long long)j * nao + i];
+        const double* row = B + ((long long)i * nao + j) * naux;
+#pragma unroll
+        for (int k = 0; k < DF_CPT; ++k) {
+            const int c = c0 + k * 256;
+            if (c < naux) acc[k] += w * row[c];
+        }
+        if (++j > i) {
+            ++i;
+            j = 0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < DF_CPT; ++k) {
+        const int c = c0 + k * 256;
+        if (c < naux) atomicAdd(rho + c, acc[k]);
+    }
+}
+
+// J_ij = J_ji = sum_P B[ij][P] rho[P] for the lower-triangle rows; one wave per
+// row, rho staged in LDS (naux doubles)
+__global__ void __launch_bounds__(256) k_df_j(const double* __restrict__ B, int nao, int naux,
+                                              const double* __restrict__ rho, double* __restrict__ J)
+{
+    extern __shared__ double srho[];
+    for (int c = threadIdx.x; c < naux; c += blockDim.x) srho[c] = rho[c];
+    __syncthreads();
+    const long long ntri = (long long)nao * (nao + 1) / 2;
+    const int lane = threadIdx.x & 63;
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < ntri; t += nw) {
+        int i, j;
+        tri_index(t, i, j);
+        const double* row = B + ((long long)i * nao + j) * naux;
+        double v = 0.0;
+        for (int c = lane; c < naux; c += 64) v += row[c] * srho[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) {
+            J[(long long)i * nao + j] = v;
+            J[(long long)j * nao + i] = v;
+        }
+    }
 }
 
 template <typename T>
@@ -479,10 +559,6 @@ int get_jk_dev(mjk_df* df, const double* d_dms, int n_dm, const double* d_orbo, 
     {
         size_t need = 0;
         RB(ctx, rocblas_start_device_memory_size_query(h));
-        (void)rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, naux, n_dm, (rocblas_int)nn, &one,
-                            df->B, naux, d_dms, (rocblas_int)nn, &zero, df->rho, naux);
-        (void)rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)nn, n_dm, naux, &one,
-                            df->B, naux, df->rho, naux, &zero, d_vj, (rocblas_int)nn);
         if (ncol > 0) {
             (void)rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, naux, ncol, nao,
                                                 &one, df->B, naux, (rocblas_stride)naux * nao, d_dms, ncol, 0, &zero,
@@ -495,11 +571,18 @@ int get_jk_dev(mjk_df* df, const double* d_dms, int n_dm, const double* d_orbo, 
     }
     if (with_j) {
         if ((rc = grow(ctx, &df->rho, &df->caprho, (size_t)naux * n_dm))) return rc;
-        // rho (naux x n_dm) = B (naux x nn) D (nn x n_dm); J (nn x n_dm) = B^T rho
-        RB(ctx, rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, naux, n_dm, (rocblas_int)nn, &one,
-                              df->B, naux, d_dms, (rocblas_int)nn, &zero, df->rho, naux));
-        RB(ctx, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)nn, n_dm, naux, &one,
-                              df->B, naux, df->rho, naux, &zero, d_vj, (rocblas_int)nn));
+        HC(ctx, hipMemsetAsync(df->rho, 0, sizeof(double) * (size_t)naux * n_dm, st));
+        const long long ntri = (long long)nao * (nao + 1) / 2;
+        const dim3 grid_rho((unsigned)((ntri + DF_ROWS - 1) / DF_ROWS), (unsigned)((naux + DF_COLS - 1) / DF_COLS));
+        const size_t lds = sizeof(double) * naux;
+        if (lds > 160 * 1024) return ctx_fail(ctx, MJK_EUNSUPPORTED, "mjk_df: %d fitting functions exceed LDS", naux);
+        if (lds > 64 * 1024)
+            HC(ctx, hipFuncSetAttribute((const void*)k_df_j, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        for (int k = 0; k < n_dm; ++k) {
+            k_df_rho<<<grid_rho, 256, 0, st>>>(df->B, nao, naux, d_dms + (size_t)k * nn, df->rho + (size_t)k * naux);
+            k_df_j<<<2048, 256, lds, st>>>(df->B, nao, naux, df->rho + (size_t)k * naux, d_vj + (size_t)k * nn);
+        }
+        HC(ctx, hipGetLastError());
     }
     if (with_k) {
         if ((rc = grow(ctx, &df->Y, &df->capY, (size_t)nao * naux * std::max(ncol, 1)))) return rc;
