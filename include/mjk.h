@@ -112,9 +112,14 @@ int mjk_basis_upload_batch(mjk_ctx* ctx, int nmol, const int* natm, const int* n
 void mjk_basis_destroy(mjk_basis* basis);
 
 /* Batched SCF (no reference counterpart): active[m] = 0 drops molecule m's
- * quartets from the following builds of the handle at block level (its J
- * and K come back zero); 1 keeps them.  All molecules start active.  The
- * reference converges one molecule per SCF loop (pyscf/scf/hf.py:171-208). */
+ * quartets from the following builds of the handle; 1 keeps them.  All
+ * molecules start active.  The row kernels' tasks of the active molecules
+ * are compacted on the device before each build and run as persistent
+ * launches (blocks claim tasks from a counter), so an iteration's cost
+ * follows its active molecules.  mjk_get_jk then copies only the active
+ * molecules' blocks: the inactive ones' blocks of dms are not read and their
+ * blocks of vj/vk are not written (mjk_get_jk_device writes them as zero).
+ * The reference converges one molecule per SCF loop (pyscf/scf/hf.py:171-208). */
 int mjk_basis_set_active(mjk_ctx* ctx, mjk_basis* basis, const int* active);
 
 /* nao of molecule `mol` (spherical AOs). */

@@ -126,8 +126,14 @@ def gc_key(k):
     return ((((((la * 4 + lb) * 4 + lc) * 4 + ld) * 2 + ca - 1) * 2 + cb - 1) * 2 + cc - 1) * 2 + cd - 1
 
 
+ROW_TCTL = 8   # ints per class of the persistent row launches (jk_common.h): count + 7 slice counters
+
+
 def generate():
     os.makedirs(BUILD, exist_ok=True)
+    for c in quartet_classes():
+        if use_row(c) and jk_split(*c) > ROW_TCTL - 1:
+            raise RuntimeError('row class %s has more slices than persistent-launch counters' % (c,))
     files = []
     head = '// generated by pyscf_amd/build.py\n#include "jk_device.h"\nnamespace mjk {\n'
     tail = '}  // namespace mjk\n'

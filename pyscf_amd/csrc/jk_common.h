@@ -29,6 +29,7 @@ constexpr int LMAX = 3;                       // s, p, d, f
 constexpr int NPAIRCLASS = (LMAX + 1) * (LMAX + 2) / 2;
 constexpr int TILE = 8;                       // quartet tile: 8 bra x 8 ket pairs
 constexpr int ROW_MAXBUCKET = 32;             // primitive-count buckets a row-kernel block can hold
+constexpr int ROW_TCTL = 8;                   // ints per class of the persistent launches: count, 7 slice heads
 
 __host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 __host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
@@ -109,6 +110,14 @@ struct JKArgs {
     const int* pmol;        // molecule of every pair
     int nbucket;
     const int* mol_active;  // per molecule: 0 skips its quartets (batched SCF; null: all active)
+    // persistent row launch (thead != null): the blocks of slice kernel S claim
+    // task entries ti = atomicAdd(thead + S, 1) until ti >= the task count
+    // (*tcount, or ntask when tcount is null); entry ti is row task tlist[ti]
+    // (ti when tlist is null).  Otherwise block b runs row task b.
+    const int* tlist;
+    const int* tcount;
+    int* thead;
+    int ntask;
 };
 
 struct QCondArgs {

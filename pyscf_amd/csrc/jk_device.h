@@ -1076,10 +1076,27 @@ void k_jk_row(const JKArgs a)
     __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
     __shared__ int s_next;   // next unclaimed ket of the chunk (waves take 64 at a time)
     const int tid = threadIdx.x, lane = tid & 63;
-    const int4 task = uload(a.rt + blockIdx.x);
+    __shared__ int s_task;
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    const bool persist = a.thead != nullptr;
+    const int ntask = persist ? (a.tcount ? *a.tcount : a.ntask) : 0;
+    for (int it = 0;; ++it) {
+    int tb;
+    if (persist) {
+        if (tid == 0) s_task = atomicAdd(a.thead + S, 1);   // each slice kernel has its own counter
+        __syncthreads();
+        const int ti = __builtin_amdgcn_readfirstlane(s_task);
+        __syncthreads();
+        if (ti >= ntask) break;   // block-uniform
+        tb = a.tlist ? uload(a.tlist + ti) : ti;
+    } else {
+        if (it > 0) break;
+        tb = blockIdx.x;
+    }
+    const int4 task = uload(a.rt + tb);
     const int p = task.x;
     const int mol = uload(a.pmol + p);
-    if (a.mol_active && !uload(a.mol_active + mol)) return;   // block-uniform
+    if (a.mol_active && !uload(a.mol_active + mol)) continue;   // block-uniform
     const int ld = uload(a.M.ld + mol);
     const long long base = uload(a.M.dbase + mol);
     const int krow = NROW * ld;
@@ -1110,14 +1127,13 @@ void k_jk_row(const JKArgs a)
     c.pvalid = true;
     c.rA = uload(a.P.rA + p);
     c.AB = uload(a.P.rAB + p);
-    const int pbo = uload(a.rtp + blockIdx.x);   // this task's share of the bra's primitive pairs
+    const int pbo = uload(a.rtp + tb);   // this task's share of the bra's primitive pairs
     c.pb0 = uload(a.P.prim0 + p) + pbo;
     c.npb = min(a.rprim, uload(a.P.nprim + p) - pbo);
     const unsigned long long quartet_tally = pbo == 0;   // a quartet counts once, in its first share
     const bool diagA = uload(a.P.shA + p) == uload(a.P.shB + p);
     const int oa = uload(a.P.cartA + p), ob = uload(a.P.cartB + p);
     const int cp = xcc_copy(a.ncopy);
-    unsigned long long cnt[4] = {0, 0, 0, 0};
     for (;;) {
         int i0 = 0;
         if (lane == 0) i0 = atomicAdd(&s_next, 64);
@@ -1188,6 +1204,8 @@ void k_jk_row(const JKArgs a)
         const int s = e / (NA * NB);
         const int ia = (e / NB) % NA, ib2 = e % NB;
         atomic_add(a.J + cp * a.jcopy + s * a.slot + base + (long long)(oa + ia) * ld + ob + ib2, v);
+    }
+    __syncthreads();   // the next task rewrites the LDS rows
     }
     if (S == 0 && a.counters) {   // one atomic per counter and block
         __shared__ unsigned long long s_cnt[4];
@@ -1492,10 +1510,27 @@ __global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
     __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
     __shared__ int s_next;
     const int tid = threadIdx.x, lane = tid & 63;
-    const int4 task = uload(a.rt + blockIdx.x);
+    __shared__ int s_task;
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    const bool persist = a.thead != nullptr;
+    const int ntask = persist ? (a.tcount ? *a.tcount : a.ntask) : 0;
+    for (int it = 0;; ++it) {
+    int tb;
+    if (persist) {
+        if (tid == 0) s_task = atomicAdd(a.thead, 1);
+        __syncthreads();
+        const int ti = __builtin_amdgcn_readfirstlane(s_task);
+        __syncthreads();
+        if (ti >= ntask) break;   // block-uniform
+        tb = a.tlist ? uload(a.tlist + ti) : ti;
+    } else {
+        if (it > 0) break;
+        tb = blockIdx.x;
+    }
+    const int4 task = uload(a.rt + tb);
     const int p = task.x;
     const int mol = uload(a.pmol + p);
-    if (a.mol_active && !uload(a.mol_active + mol)) return;   // block-uniform
+    if (a.mol_active && !uload(a.mol_active + mol)) continue;   // block-uniform
     const int ld = uload(a.M.ld + mol);
     const long long base = uload(a.M.dbase + mol);
     const int krow = NROW * ld;
@@ -1524,7 +1559,7 @@ __global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
     c.pvalid = true;
     c.rA = uload(a.P.rA + p);
     c.AB = uload(a.P.rAB + p);
-    const int pbo = uload(a.rtp + blockIdx.x);
+    const int pbo = uload(a.rtp + tb);
     c.pb0 = uload(a.P.prim0 + p) + pbo;
     c.npb = min(a.rprim, uload(a.P.nprim + p) - pbo);
     // a merged quartet counts as its NCOMB quartets of segmented shells, once
@@ -1532,7 +1567,6 @@ __global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
     const bool diagA = uload(a.P.shA + p) == uload(a.P.shB + p);
     const int oa = uload(a.P.cartA + p), ob = uload(a.P.cartB + p);
     const int cp = xcc_copy(a.ncopy);
-    unsigned long long cnt[4] = {0, 0, 0, 0};
     for (;;) {
         int i0 = 0;
         if (lane == 0) i0 = atomicAdd(&s_next, 64);
@@ -1603,6 +1637,8 @@ __global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
         const int s = e / (NRA * NRB);
         const int ra = (e / NRB) % NRA, rb = e % NRB;
         atomic_add(a.J + cp * a.jcopy + s * a.slot + base + (long long)(oa + ra) * ld + ob + rb, v);
+    }
+    __syncthreads();   // the next task rewrites the LDS rows
     }
     if (a.counters) {   // one atomic per counter and block
         __shared__ unsigned long long s_cnt[4];

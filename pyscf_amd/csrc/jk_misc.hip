@@ -246,6 +246,34 @@ __global__ __launch_bounds__(256) void k_zero(double* __restrict__ dst, long lon
 
 // ---------------------------------------------------------------------------
 // Host-side launchers.
+// Row tasks of active molecules: list[count++] = t for every task t whose bra
+// pair belongs to an active molecule (one wave-aggregated atomic per wave;
+// the order within a wave is kept).  count must be zero before the launch.
+__global__ __launch_bounds__(256) void k_compact_rows(const int4* __restrict__ rt, int nrt,
+                                                       const int* __restrict__ pmol, const int* __restrict__ active,
+                                                       int* __restrict__ list, int* __restrict__ count)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = t < nrt && active[pmol[rt[t].x]] != 0;
+    const unsigned long long mask = __ballot(on);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    const int leader = __ffsll((long long)mask) - 1;
+    if (lane == leader) base = atomicAdd(count, __popcll(mask));
+    base = __shfl(base, leader);
+    if (on) list[base + __popcll(mask & ((1ull << lane) - 1))] = t;
+}
+
+int launch_compact_rows(const int4* rt, int nrt, const int* pmol, const int* active, int* list, int* count,
+                        hipStream_t st)
+{
+    if (nrt <= 0) return 0;
+    hipLaunchKernelGGL(k_compact_rows, dim3((unsigned)((nrt + 255) / 256)), dim3(256), 0, st, rt, nrt, pmol, active,
+                       list, count);
+    return (int)hipGetLastError();
+}
+
 int launch_zero(double* dst, long long n, hipStream_t st)
 {
     if (n <= 0) return 0;

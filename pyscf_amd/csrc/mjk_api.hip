@@ -57,6 +57,8 @@ int launch_sph2cart(const XformArgs& x, int maxld, const double* Dsph, long long
 int launch_cart2sph(const XformArgs& x, int maxnao, const double* Jc, const double* Kc, long long cslot,
                     int n_dm, int hermi, int with_j, int with_k, double* vj, double* vk, long long sslot,
                     int ncopy, long long jcopy, long long kcopy, hipStream_t st);
+int launch_compact_rows(const int4* rt, int nrt, const int* pmol, const int* active, int* list, int* count,
+                        hipStream_t st);
 int launch_dmcond(const DmCondArgs& x, const double* Dsph, long long sslot, int n_dm, double* out,
                   hipStream_t st);
 int lmax_kernels();
@@ -156,6 +158,7 @@ struct QClass {
     int rprim = 0;           // bra primitive pairs per task
     int bucket = 0;          // floor(log2) of the Schwarz-prescreened primitive quartets (tuning key)
     int kernel_pref = 0;     // tuning: 0 the default choice, 1 row kernel, 2 fallback (tile / cooperative)
+    long long tl0 = 0;       // offset of this class's segment of the compacted task lists
 };
 
 
@@ -219,6 +222,11 @@ struct mjk_ctx {
     int sched = 1;                        // class cost for the stream assignment: 0 work, 1 est; MJK_SCHED
     int graphs = 1;                       // replay plain builds as HIP graphs (mjk_ctx_set_graphs, MJK_GRAPHS)
     int order = 1;                        // issue order within a stream: 0 by cost, 1 heavy classes first; MJK_ORDER
+    // persistent row launches (blocks claim tasks from a device counter): 0 never,
+    // 1 batched bases (the tasks of inactive molecules are compacted away on the
+    // device first), 2 every basis; MJK_PERSIST
+    int persist = 1;
+    int persist_blocks = 2048;            // blocks of a persistent row launch; MJK_PERSIST_BLOCKS
     double expcutoff = EXPCUTOFF_DEFAULT; // primitive pairs with exp(-ab/p AB^2) < e^-expcutoff dropped; MJK_EXPCUTOFF
     hipStream_t aux[NAUX] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr;
@@ -415,6 +423,11 @@ struct mjk_basis {
     std::vector<int> seg;            // [(group * nmol + mol) * NBUCKET + bucket] -> first pair, group = class * NGT + type
     int* d_pmol = nullptr;           // molecule of every pair
     int* d_active = nullptr;         // per molecule: 0 skips its quartets (mjk_basis_set_active)
+    std::vector<int> h_active;       // host copy of d_active
+    std::vector<long long> h_sbase;  // per molecule: spherical arena offset (block of nao^2)
+    std::vector<int> h_nao;
+    int* d_tlist = nullptr;          // compacted row-task lists, one segment per class (QClass::tl0)
+    int* d_tctl = nullptr;           // per class ROW_TCTL ints: task count, one claim counter per slice kernel
     int class_filter = -1;           // >= 0: builds run only this quartet class (mjk_basis_set_class_filter)
     int tune_gen = -1;               // ctx->tune_gen the task lists were built with
     std::vector<double> q;           // Schwarz bound per pair (sorted order)
@@ -466,6 +479,10 @@ void free_tiles(mjk_basis* b)
         dfree(b->ctx, c.rr);
         dfree(b->ctx, c.rtp);
     }
+    dfree(b->ctx, b->d_tlist);
+    dfree(b->ctx, b->d_tctl);
+    b->d_tlist = nullptr;
+    b->d_tctl = nullptr;
     b->qcls.clear();
     b->tile_cutoff = -1.0;
 }
@@ -729,6 +746,15 @@ int build_tiles(mjk_basis* b, double cutoff)
             }
         }
     }
+    // compacted task lists and the persistent launches' counters
+    long long ntl = 0;
+    for (auto& qc : b->qcls) {
+        qc.tl0 = ntl;
+        ntl += qc.nrt;
+    }
+    int rc2;
+    if (ntl > 0 && (rc2 = dalloc(ctx, sizeof(int) * (size_t)ntl, (void**)&b->d_tlist))) return rc2;
+    if ((rc2 = dalloc(ctx, sizeof(int) * ROW_TCTL * NQCLASS_MAX, (void**)&b->d_tctl))) return rc2;
     b->tile_cutoff = cutoff;
     b->tune_gen = ctx->tune_gen;
     return MJK_OK;
@@ -1228,6 +1254,9 @@ int build_basis(mjk_ctx* ctx, mjk_basis* b, double cutoff)
     if ((rc = track_upload(b, dbase, &b->d_dbase))) return rc;
     if ((rc = track_upload(b, cbase, &b->d_cbase))) return rc;
     if ((rc = track_upload(b, sbase, &b->d_sbase))) return rc;
+    b->h_sbase = sbase;
+    b->h_nao = nao;
+    b->h_active.assign(b->nmol, 1);
     if ((rc = track_upload(b, cfirst, &b->d_cart_first))) return rc;
     if ((rc = track_upload(b, sfirst, &b->d_sph_first))) return rc;
     if ((rc = track_upload(b, hfirst, &b->d_sh_first))) return rc;
@@ -1431,6 +1460,8 @@ int mjk_ctx_create(mjk_ctx** out, int device, size_t hbm_budget)
     if (const char* sc = getenv("MJK_SCHED")) ctx->sched = atoi(sc);
     if (const char* gr = getenv("MJK_GRAPHS")) ctx->graphs = atoi(gr) != 0;
     if (const char* od = getenv("MJK_ORDER")) ctx->order = atoi(od);
+    if (const char* pe = getenv("MJK_PERSIST")) ctx->persist = std::max(0, std::min(2, atoi(pe)));
+    if (const char* pb = getenv("MJK_PERSIST_BLOCKS")) ctx->persist_blocks = std::max(64, std::min(1 << 20, atoi(pb)));
     if (const char* ec = getenv("MJK_EXPCUTOFF")) ctx->expcutoff = std::max(20.0, std::min(EXPCUTOFF_MAX, atof(ec)));
     for (auto& e : ctx->ev) (void)hipEventCreate(&e);
     bool streams_ok = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) == hipSuccess;
@@ -1575,6 +1606,7 @@ int mjk_basis_set_active(mjk_ctx* ctx, mjk_basis* b, const int* active)
     // stream-ordered before the builds that follow (a replayed graph reads the same buffer)
     HIPCHK(ctx, hipMemcpyAsync(b->d_active, act.data(), sizeof(int) * b->nmol, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    b->h_active = act;
     return MJK_OK;
 }
 
@@ -1721,6 +1753,16 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
     a.pmol = b->d_pmol;
     a.nbucket = NBUCKET;
     a.mol_active = b->nmol > 1 ? b->d_active : nullptr;
+    a.tlist = nullptr;
+    a.tcount = nullptr;
+    a.thead = nullptr;
+    a.ntask = 0;
+    // persistent row launches: the blocks claim tasks from a counter; a batched
+    // basis first compacts the tasks of its active molecules into a list
+    const bool persist = ctx->persist == 2 || (ctx->persist == 1 && b->nmol > 1);
+    const bool compact = persist && b->nmol > 1;
+    if (persist && launch_zero(reinterpret_cast<double*>(b->d_tctl), ROW_TCTL / 2 * NQCLASS_MAX, st))
+        return fail(ctx, MJK_EDEVICE, "zero fill launch failed");
     // classes, heaviest first, each on the least loaded of the streams (they all
     // add into the same J'/K' with atomics, so their order does not matter)
     const int nst = ctx->nstreams;
@@ -1766,13 +1808,32 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
         a.nbt = qc.nbt;
         a.diag = qc.GX == qc.GY;
         a.counters = tally ? b->counters + 4 * c : nullptr;   // work tallies only on request
+        // the row launches' size: one block per task, or the persistent grid
+        int nrow_blocks = qc.nrt;
+        if (persist && qc.nrt > 0) {
+            a.thead = b->d_tctl + ROW_TCTL * c + 1;
+            a.ntask = qc.nrt;
+            a.tcount = compact ? b->d_tctl + ROW_TCTL * c : nullptr;
+            a.tlist = compact ? b->d_tlist + qc.tl0 : nullptr;
+            nrow_blocks = std::min(qc.nrt, ctx->persist_blocks);
+        } else {
+            a.thead = nullptr;
+            a.tcount = nullptr;
+            a.tlist = nullptr;
+            a.ntask = 0;
+        }
+        const bool row_class = qc.mode == 2 || (row_shmem(qc, n_j, n_k, b->maxld) <= ctx->row_lds && qc.nrt > 0 &&
+                                                qc.kernel_pref != 2);
+        if (compact && row_class && qc.nrt > 0 &&
+            launch_compact_rows(qc.rt, qc.nrt, b->d_pmol, b->d_active, b->d_tlist + qc.tl0, b->d_tctl + ROW_TCTL * c, st))
+            return fail(ctx, MJK_EDEVICE, "task compaction launch failed");
         if (qc.mode == 2) {
             // merged classes: the merged row kernel (its LDS fit was checked for this build)
             a.rt = qc.rt;
             a.rr = qc.rr;
             a.rtp = qc.rtp;
             a.rprim = qc.rprim;
-            if (launch_jk_rows_gc(qc.la, qc.lb, qc.lc, qc.ld, qc.ca, qc.cb, qc.cc, qc.cd, qc.nrt, a,
+            if (launch_jk_rows_gc(qc.la, qc.lb, qc.lc, qc.ld, qc.ca, qc.cb, qc.cc, qc.cd, nrow_blocks, a,
                                   row_shmem(qc, n_j, n_k, b->maxld), st))
                 return fail(ctx, MJK_EDEVICE, "merged row kernel launch failed for class (%d%d|%d%d)", qc.la, qc.lb,
                             qc.lc, qc.ld);
@@ -1788,7 +1849,7 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
             a.rr = qc.rr;
             a.rtp = qc.rtp;
             a.rprim = qc.rprim;
-            rr = launch_jk_rows(qc.la, qc.lb, qc.lc, qc.ld, qc.nrt, a, shmem, st);
+            rr = launch_jk_rows(qc.la, qc.lb, qc.lc, qc.ld, nrow_blocks, a, shmem, st);
             if (rr > 0) return fail(ctx, MJK_EDEVICE, "row kernel launch failed for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
         }
         if (rr != 0 && launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
@@ -1940,20 +2001,67 @@ int mjk_get_jk(mjk_ctx* ctx, mjk_basis* b, const double* dms, int n_dm, int herm
     auto t0 = std::chrono::steady_clock::now();
     int rc;
     std::lock_guard<std::recursive_mutex> outer(ctx->mu);
+    // A batch with inactive molecules moves only the active molecules' blocks
+    // over PCIe (runs of consecutive active molecules, one copy per run and
+    // density): the inactive molecules' blocks of dms are not read, and their
+    // blocks of vj/vk are not written.
+    std::vector<std::pair<long long, long long>> runs;   // {offset, length} within a slot
+    {
+        const int nm = b->nmol;
+        bool partial = false;
+        for (int m = 0; m < nm && nm > 1; ++m) partial = partial || !b->h_active[m];
+        if (partial) {
+            for (int m = 0; m < nm;) {
+                if (!b->h_active[m]) {
+                    ++m;
+                    continue;
+                }
+                int e = m;
+                while (e < nm && b->h_active[e]) ++e;
+                const long long lo = b->h_sbase[m];
+                const long long hi = b->h_sbase[e - 1] + (long long)b->h_nao[e - 1] * b->h_nao[e - 1];
+                runs.push_back({lo, hi - lo});
+                m = e;
+            }
+        } else {
+            runs.push_back({0, b->slot_s});
+        }
+    }
+    const bool partial = !(runs.size() == 1 && runs[0].first == 0 && runs[0].second == b->slot_s);
     {
         HIPCHK(ctx, hipSetDevice(ctx->device));
         const long long n = (long long)n_dm * b->slot_s;
         if ((rc = ensure(ctx, &b->Dsph, b->cap_ds, n))) return rc;
         if (with_j && (rc = ensure(ctx, &b->Vj, b->cap_vj, n))) return rc;
         if (with_k && (rc = ensure(ctx, &b->Vk, b->cap_vk, n))) return rc;
-        HIPCHK(ctx, hipMemcpyAsync(b->Dsph, dms, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        if (!partial) {
+            HIPCHK(ctx, hipMemcpyAsync(b->Dsph, dms, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            for (int s = 0; s < n_dm; ++s)
+                for (const auto& r : runs)
+                    HIPCHK(ctx, hipMemcpyAsync(b->Dsph + s * b->slot_s + r.first, dms + s * b->slot_s + r.first,
+                                               sizeof(double) * r.second, hipMemcpyHostToDevice, ctx->stream));
+        }
     }
     rc = mjk_get_jk_device(ctx, b, b->Dsph, n_dm, hermi, with_j, with_k, direct_scf_tol, fp32_tol,
                            with_j ? b->Vj : nullptr, with_k ? b->Vk : nullptr, stats);
     if (rc) return rc;
     const long long n = (long long)n_dm * b->slot_s;
-    if (with_j) HIPCHK(ctx, hipMemcpyAsync(vj, b->Vj, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
-    if (with_k) HIPCHK(ctx, hipMemcpyAsync(vk, b->Vk, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (!partial) {
+        if (with_j) HIPCHK(ctx, hipMemcpyAsync(vj, b->Vj, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+        if (with_k) HIPCHK(ctx, hipMemcpyAsync(vk, b->Vk, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    } else {
+        for (int s = 0; s < n_dm; ++s)
+            for (const auto& r : runs) {
+                const long long o = s * b->slot_s + r.first;
+                if (with_j)
+                    HIPCHK(ctx, hipMemcpyAsync(vj + o, b->Vj + o, sizeof(double) * r.second, hipMemcpyDeviceToHost,
+                                               ctx->stream));
+                if (with_k)
+                    HIPCHK(ctx, hipMemcpyAsync(vk + o, b->Vk + o, sizeof(double) * r.second, hipMemcpyDeviceToHost,
+                                               ctx->stream));
+            }
+    }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (stats)
         stats->ms_total =

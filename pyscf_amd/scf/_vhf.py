@@ -156,8 +156,9 @@ class Basis:
         if dms.size != n_dm * self.slot:
             raise ValueError('density shape %s does not match the basis (%d elements per slot)'
                              % (dms.shape, self.slot))
-        vj = numpy.empty_like(dms) if with_j else None
-        vk = numpy.empty_like(dms) if with_k else None
+        # zero-filled: with inactive molecules the library writes only the active blocks
+        vj = numpy.zeros_like(dms) if with_j else None
+        vk = numpy.zeros_like(dms) if with_k else None
         st = _lib.mjk_stats()
         rc = self.engine.lib.mjk_get_jk(self.engine.handle, self.handle, _p(dms), n_dm, int(hermi),
                                         int(bool(with_j)), int(bool(with_k)), float(direct_scf_tol),
