@@ -460,7 +460,8 @@ def bench_batched(args):
 
 def bench_serving(args):
     """Poisson stream of get_jk and RHF requests (50-max AOs, lognormal;
-    70% def2-SVP, 30% def2-TZVP) through the continuous-batching server."""
+    70% def2-SVP, 30% def2-TZVP; the first request at the largest size, in
+    def2-TZVP) through the continuous-batching server."""
     from pyscf_amd import gto, workloads
     from pyscf_amd.server import JKServer, Request
     rng = numpy.random.default_rng(5)
@@ -468,6 +469,8 @@ def bench_serving(args):
     for i in range(args.nreq):
         basis = 'def2-svp' if rng.random() < 0.7 else 'def2-tzvp'
         target = float(numpy.clip(rng.lognormal(numpy.log(200), 0.8), 50, args.max_ao))
+        if i == 0:          # the stream always holds one request at the largest size
+            basis, target = 'def2-tzvp', float(args.max_ao)
         per_heavy = 14 if basis == 'def2-svp' else 31
         nheavy = max(2, int(round(target / (per_heavy + 8))))
         mol = gto.M(atom=workloads.random_chain([5, i], nmin=nheavy, nmax=nheavy), basis=basis)

@@ -227,6 +227,14 @@ int mjk_rhf_fetch(mjk_ctx* ctx, mjk_scf* scf, double* dm, double* mo_energy, dou
 const double* mjk_rhf_device_density(const mjk_scf* scf);
 void mjk_rhf_destroy(mjk_scf* scf);
 
+/* One-electron integrals on the device (mol.intor('int1e_ovlp'),
+ * 'int1e_kin', 'int1e_nuc' of pyscf/gto/moleintor.py; the core Hamiltonian
+ * of pyscf/scf/hf.py:315-345): any of s, t, v (nao x nao, spherical) may be
+ * NULL; device_out = 1 when they are device pointers (asynchronous then on the
+ * context's stream), 0 for host arrays.  Point nuclei; shells up to f. */
+int mjk_int1e_device(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas, const double* env, double* s,
+                     double* t, double* v, int device_out);
+
 /* Density-fitted J/K (pyscf/df/incore.py:129-214 cholesky_eri; pyscf/df/df_jk.py:278-418
  * get_jk), Coulomb operator.
  *   mjk_df_build: the orbital basis (atm/bas/env, shells up to f) and the

@@ -24,7 +24,7 @@ FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC',
          '-munsafe-fp-atomics', '-Wno-unused-result', '-I' + CSRC, '-I' + INCLUDE]
 # dense algebra of the device-resident SCF
 LIBS = ['-L/opt/rocm/lib', '-lrocsolver', '-lrocblas', '-Wl,-rpath,/opt/rocm/lib']
-STATIC_SOURCES = ['jk_misc.hip', 'mjk_api.hip', 'mjk_scf.hip', 'mjk_serve.hip', 'mjk_df.hip']
+STATIC_SOURCES = ['jk_misc.hip', 'mjk_api.hip', 'mjk_scf.hip', 'mjk_serve.hip', 'mjk_df.hip', 'mjk_int1e.hip']
 LMAX = 3
 
 

@@ -174,10 +174,10 @@ std::string execute(mjk_ctx* ctx, Req& r, size_t idle)
         if (rc) err = ctx_error(ctx, rc, "mjk_get_jk");
     } else {
         std::vector<double> s1e(nn), t(nn), v(nn);
-        if (mjk_int1e(0, r.atm.data(), r.natm, r.bas.data(), r.nbas, r.env.data(), s1e.data()) ||
-            mjk_int1e(1, r.atm.data(), r.natm, r.bas.data(), r.nbas, r.env.data(), t.data()) ||
-            mjk_int1e(2, r.atm.data(), r.natm, r.bas.data(), r.nbas, r.env.data(), v.data())) {
-            err = "mjk_int1e failed";
+        rc = mjk_int1e_device(ctx, r.atm.data(), r.natm, r.bas.data(), r.nbas, r.env.data(), s1e.data(), t.data(),
+                              v.data(), 0);
+        if (rc) {
+            err = ctx_error(ctx, rc, "mjk_int1e_device");
         } else {
             for (size_t i = 0; i < nn; ++i) t[i] += v[i];
             mjk_scf* scf = nullptr;

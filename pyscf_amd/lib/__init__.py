@@ -93,6 +93,8 @@ def _declare(lib):
     lib.mjk_rhf_device_density.restype = c_void_p
     lib.mjk_rhf_destroy.argtypes = [c_void_p]
     lib.mjk_rhf_destroy.restype = None
+    lib.mjk_int1e_device.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_int]
     lib.mjk_df_build.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
                                  c_int, c_void_p, c_double, P(c_void_p)]
     lib.mjk_df_naux.argtypes = [c_void_p]

@@ -70,8 +70,9 @@ class BatchRHF:
         if self.basis is None:
             self.basis = self.engine.upload_batch(mols, self.direct_scf_tol)
         drivers = [rhf.RHF(m) for m in mols]
-        h1e = [d.get_hcore() for d in drivers]
-        s1e = [d.get_ovlp() for d in drivers]
+        ints = [rhf.int1e_device(m, self.engine) for m in mols]   # on the device, one call per molecule
+        h1e = [x['kin'] + x['nuc'] for x in ints]
+        s1e = [x['ovlp'] for x in ints]
         enuc = [m.energy_nuc() for m in mols]
         dms = []
         for d, h, s in zip(drivers, h1e, s1e):

@@ -42,6 +42,20 @@ def int1e(mol, which):
     return out
 
 
+def int1e_device(mol, engine=None, which=('ovlp', 'kin', 'nuc')):
+    """The same matrices computed on the GPU (mjk_int1e_device), as a dict."""
+    eng = engine or _vhf.get_engine()
+    atm = numpy.ascontiguousarray(mol._atm, dtype=numpy.int32)
+    bas = numpy.ascontiguousarray(mol._bas, dtype=numpy.int32)
+    env = numpy.ascontiguousarray(mol._env, dtype=numpy.double)
+    nao = int(mol.nao_nr())
+    out = {w: numpy.empty((nao, nao)) for w in which}
+    p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+    eng.check(eng.lib.mjk_int1e_device(eng.handle, p(atm), len(atm), p(bas), len(bas), p(env), p(out.get('ovlp')),
+                                       p(out.get('kin')), p(out.get('nuc')), 0), 'mjk_int1e_device')
+    return out
+
+
 class CDIIS:
     """Commutator DIIS on the Fock matrix."""
 
@@ -227,8 +241,6 @@ class DeviceRHF(RHF):
         mol = self.mol
         if mol.nelectron % 2:
             raise ValueError('RHF needs an even number of electrons')
-        h1e = numpy.ascontiguousarray(self.get_hcore())
-        s1e = numpy.ascontiguousarray(self.get_ovlp())
         enuc = float(mol.energy_nuc())
         conv_tol_grad = self.conv_tol_grad or numpy.sqrt(self.conv_tol)
         if self._opt is None:
@@ -236,6 +248,9 @@ class DeviceRHF(RHF):
         if self._opt.get(None) is None:
             self._opt[None] = self.init_direct_scf(mol)
         opt = self._opt[None]
+        ints = int1e_device(mol, opt.engine)      # one-electron integrals on the device too
+        h1e = numpy.ascontiguousarray(ints['kin'] + ints['nuc'])
+        s1e = ints['ovlp']
         eng, L = opt.engine, opt.engine.lib
         nao = s1e.shape[0]
         p = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
