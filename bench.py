@@ -599,6 +599,12 @@ def bench_df(args):
     t0 = time.perf_counter()
     e_df = mf.kernel()
     t_scf = time.perf_counter() - t0
+    progress('df: device-resident DF-RHF')
+    mfd = rhf.DeviceRHF(mol, direct_scf_tol=TOL).density_fit(with_df=d)
+    mfd.conv_tol = 1e-10
+    t0 = time.perf_counter()
+    e_dfd = mfd.kernel()
+    t_scf_dev = time.perf_counter() - t0
     progress('df: exact RHF for the fitting error')
     mfx = rhf.RHF(mol, direct_scf_tol=TOL)
     mfx.conv_tol = 1e-10
@@ -660,7 +666,8 @@ def bench_df(args):
                                  'tflops': flops_gen / t_gen * 1e-12},
         'fit': {'seconds': t_fit, 'naux': naux, 'tensor_gb': 8e-9 * naux * nn},
         'scf': {'e_df': e_df, 'e_exact': e_exact, 'fitting_error_eh': e_df - e_exact, 'cycles': mf.cycles,
-                'seconds': t_scf},
+                'seconds': t_scf, 'device_resident': {'e_df': e_dfd, 'cycles': mfd.cycles, 'seconds': t_scf_dev,
+                                                      'dE_vs_host_loop': e_dfd - e_df}},
         'paths_agree': {'max_dJ': float(abs(vj - vj2).max()), 'max_dK': float(abs(vk - vk2).max())},
         'roofline': {'bound': 'mfma', 'achieved': flops_mo / t_mo * 1e-12, 'peak': FP64_PEAK * 1e-12,
                      'unit': 'TFLOP/s', 'frac': flops_mo / t_mo / FP64_PEAK, 'traffic': None,

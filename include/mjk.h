@@ -225,6 +225,14 @@ int mjk_rhf_run(mjk_ctx* ctx, mjk_scf* scf, const double* dm0, int max_cycle, do
                 double direct_scf_tol, double fp32_tol, double level_shift, mjk_rhf_result* res);
 int mjk_rhf_fetch(mjk_ctx* ctx, mjk_scf* scf, double* dm, double* mo_energy, double* mo_coeff, double* fock);
 const double* mjk_rhf_device_density(const mjk_scf* scf);
+/* Density-fitted SCF (pyscf/df/df_jk.py:32-125, the SCF of density_fit()):
+ * the following runs build J and K of the full density from the fitted
+ * tensor every cycle, K from the occupied orbitals on the device; df = NULL
+ * returns to exact J/K.  The fit must belong to this context and molecule
+ * (mjk_df_build); a DF run starts from the core Hamiltonian guess.  The
+ * mjk_df declarations follow below. */
+struct mjk_df;
+int mjk_rhf_set_df(mjk_ctx* ctx, mjk_scf* scf, struct mjk_df* df);
 void mjk_rhf_destroy(mjk_scf* scf);
 
 /* One-electron integrals on the device (mol.intor('int1e_ovlp'),

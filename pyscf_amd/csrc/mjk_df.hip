@@ -508,7 +508,7 @@ int build(mjk_ctx* ctx, mjk_df* df, const int* atm, int natm, const int* bas, in
 // J and K of n_dm densities (device, nao x nao each, row-major), optional
 // occupied orbitals X (n_dm x nao x nocc, row-major) for K
 int get_jk_dev(mjk_df* df, const double* d_dms, int n_dm, const double* d_orbo, int nocc, int with_j, int with_k,
-               double* d_vj, double* d_vk)
+               double* d_vj, double* d_vk, bool orbo_colmajor = false)
 {
     mjk_ctx* ctx = df->ctx;
     hipStream_t st = mjk_internal::ctx_stream(ctx);
@@ -558,11 +558,15 @@ int get_jk_dev(mjk_df* df, const double* d_dms, int n_dm, const double* d_orbo, 
                 HC(ctx, hipMemsetAsync(K, 0, sizeof(double) * nn, st));
                 continue;
             }
-            // Y_i (naux x ncol) = B_i (naux x nao) op(M), M = D (row-major: op T) or X (row-major nao x nocc: op T)
+            // Y_i (naux x ncol) = B_i (naux x nao) op(M), M = D (row-major: op T) or X (row-major nao x nocc:
+            // op T; column-major, leading dimension nao: op N)
             const double* M = d_orbo ? d_orbo + (size_t)k * nao * nocc : d_dms + (size_t)k * nn;
-            RB(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, naux, ncol,
-                                                  nao, &one, df->B, naux, (rocblas_stride)naux * nao, M, ncol, 0,
-                                                  &zero, df->Y, naux, (rocblas_stride)naux * ncol, nao));
+            const bool cm = d_orbo && orbo_colmajor;
+            RB(ctx, rocblas_dgemm_strided_batched(h, rocblas_operation_none,
+                                                  cm ? rocblas_operation_none : rocblas_operation_transpose, naux,
+                                                  ncol, nao, &one, df->B, naux, (rocblas_stride)naux * nao, M,
+                                                  cm ? nao : ncol, 0, &zero, df->Y, naux, (rocblas_stride)naux * ncol,
+                                                  nao));
             if (d_orbo) {
                 // K = Y^T Y over (P, o)
                 RB(ctx, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nao, nao, naux * ncol,
@@ -592,6 +596,25 @@ void release(mjk_df* df)
 }
 
 }  // namespace
+
+namespace mjk_internal {
+
+int df_get_jk_dev(mjk_df* df, const double* d_dms, const double* d_orbo_colmajor, int nocc, double* d_vj, double* d_vk)
+{
+    return get_jk_dev(df, d_dms, 1, d_orbo_colmajor, nocc, 1, 1, d_vj, d_vk, true);
+}
+
+int df_nao(const mjk_df* df)
+{
+    return df->nao;
+}
+
+mjk_ctx* df_ctx(const mjk_df* df)
+{
+    return df->ctx;
+}
+
+}  // namespace mjk_internal
 
 extern "C" {
 

@@ -31,5 +31,11 @@ void ctx_set_blas(mjk_ctx* ctx, void* handle, void (*destroy)(void*));
 void* ctx_rocblas(mjk_ctx* ctx);
 // Rys root table of the context (device)
 const double* ctx_rys(const mjk_ctx* ctx);
+// density-fitted J and K of one density (device, nao x nao) with K from its
+// occupied orbitals X (device, column-major nao x nocc, D = X X^T)
+int df_get_jk_dev(mjk_df* df, const double* d_dm, const double* d_orbo_colmajor, int nocc, double* d_vj,
+                  double* d_vk);
+int df_nao(const mjk_df* df);
+mjk_ctx* df_ctx(const mjk_df* df);
 
 }  // namespace mjk_internal

@@ -64,6 +64,10 @@ struct mjk_scf {
     // results
     double e_tot = 0, e_last = 0, gnorm = 0;
     int cycles = 0, converged = 0;
+    // density fitting (mjk_rhf_set_df): full J/K of D every cycle, K from the
+    // occupied orbitals X = sqrt(2) C_occ
+    mjk_df* df = nullptr;
+    double* Xo = nullptr;
 };
 
 namespace {
@@ -288,11 +292,25 @@ int check_info(mjk_scf* s, const char* what)
     return MJK_OK;
 }
 
+__global__ void k_scale(long long n, double a, const double* x, double* out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a * x[i];
+}
+
 // J/K of src into V (V = J - K/2, or V += J - K/2 when accumulate)
 int veff(mjk_scf* s, const double* src, bool accumulate, double direct_scf_tol, double fp32_tol)
 {
     const long long nn = (long long)s->n * s->n;
-    int rc = mjk_get_jk_device(s->ctx, s->basis, src, 1, 1, 1, 1, direct_scf_tol, fp32_tol, s->J, s->K, nullptr);
+    int rc;
+    if (s->df) {
+        // density fitting: J and K of the full density from the orbitals (src is D)
+        const long long no = (long long)s->n * s->nocc;
+        k_scale<<<grid1(no), NT, 0, mjk_internal::ctx_stream(s->ctx)>>>(no, std::sqrt(2.0), s->C, s->Xo);
+        rc = mjk_internal::df_get_jk_dev(s->df, src, s->Xo, s->nocc, s->J, s->K);
+    } else {
+        rc = mjk_get_jk_device(s->ctx, s->basis, src, 1, 1, 1, 1, direct_scf_tol, fp32_tol, s->J, s->K, nullptr);
+    }
     if (rc) return rc;
     hipStream_t st = mjk_internal::ctx_stream(s->ctx);
     if (accumulate)
@@ -418,7 +436,7 @@ void release(mjk_scf* s)
         (void)hipStreamSynchronize(mjk_internal::ctx_stream(s->ctx));
         double** bufs[] = {&s->S, &s->H, &s->X, &s->D, &s->Dlast, &s->dD, &s->V, &s->F, &s->Fx, &s->T1,
                            &s->T2, &s->J, &s->K, &s->C, &s->Sc, &s->w, &s->e, &s->ring_f, &s->ring_e,
-                           &s->scal, &s->coef};
+                           &s->scal, &s->coef, &s->Xo};
         for (double** b : bufs)
             if (*b) (void)mjk_device_free(s->ctx, *b);
         if (s->info) (void)mjk_device_free(s->ctx, s->info);
@@ -540,6 +558,8 @@ int run(mjk_scf* s, const double* dm0, int max_cycle, double conv_tol, double co
     s->cycles = 0;
     // initial density: the caller's, or the core Hamiltonian's ('1e' guess)
     if (dm0) {
+        if (s->df)
+            return ctx_fail(s->ctx, MJK_EINVAL, "mjk_rhf_run: a density-fitted run starts from the core guess");
         HCHK(s, hipMemcpyAsync(s->D, dm0, nn * sizeof(double), hipMemcpyHostToDevice, st));
     } else {
         HCHK(s, hipMemcpyAsync(s->Fx, s->H, nn * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -556,10 +576,13 @@ int run(mjk_scf* s, const double* dm0, int max_cycle, double conv_tol, double co
         if ((rc = solve_fock(s)) || (rc = check_info(s, "Fock eigenpairs"))) return rc;
         std::swap(s->D, s->Dlast);
         if ((rc = make_density(s))) return rc;
-        k_lincomb<<<grid1(nn), NT, 0, st>>>(nn, 1.0, s->D, -1.0, s->Dlast, s->dD);
-        if ((rc = veff(s, s->dD, true, direct_scf_tol, fp32_tol)) || (rc = energy_and_fock(s)) ||
-            (rc = gradient(s)) || (rc = read_scalars(s, &e1, &e2, &g)))
-            return rc;
+        if (s->df) {   // density fitting: full builds (the reference turns direct_scf off for DF)
+            if ((rc = veff(s, s->D, false, direct_scf_tol, fp32_tol))) return rc;
+        } else {
+            k_lincomb<<<grid1(nn), NT, 0, st>>>(nn, 1.0, s->D, -1.0, s->Dlast, s->dD);
+            if ((rc = veff(s, s->dD, true, direct_scf_tol, fp32_tol))) return rc;
+        }
+        if ((rc = energy_and_fock(s)) || (rc = gradient(s)) || (rc = read_scalars(s, &e1, &e2, &g))) return rc;
         s->e_tot = e1 + 0.5 * e2 + s->enuc;
         s->gnorm = g;
         s->cycles = cycle + 1;
@@ -639,6 +662,19 @@ int mjk_rhf_fetch(mjk_ctx* ctx, mjk_scf* scf, double* dm, double* mo_energy, dou
     if (mo_coeff)   // column-major C to row-major (nao, nmo)
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < n; ++j) mo_coeff[(size_t)i * n + j] = c[(size_t)j * n + i];
+    return MJK_OK;
+}
+
+int mjk_rhf_set_df(mjk_ctx* ctx, mjk_scf* scf, mjk_df* df)
+{
+    if (!ctx || !scf || scf->ctx != ctx || (df && (mjk_internal::df_ctx(df) != ctx || mjk_internal::df_nao(df) != scf->n)))
+        return ctx_fail(ctx, MJK_EINVAL, "bad arguments to mjk_rhf_set_df (the fit must be of this context and basis)");
+    std::lock_guard<std::recursive_mutex> lock(mjk_internal::ctx_mutex(ctx));
+    if (df && !scf->Xo) {
+        const int rc = alloc(scf, &scf->Xo, (size_t)scf->n * std::max(scf->nocc, 1));
+        if (rc) return rc;
+    }
+    scf->df = df;
     return MJK_OK;
 }
 

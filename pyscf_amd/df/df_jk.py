@@ -86,8 +86,12 @@ def density_fit(mf, auxbasis=None, with_df=None):
         def make_rdm1(self, mo_coeff, mo_occ):
             return _lib.tag_array(base.make_rdm1(mo_coeff, mo_occ), mo_coeff=mo_coeff, mo_occ=mo_occ)
 
-    if base is rhf.DeviceRHF:
-        raise NotImplementedError('density fitting of the device-resident RHF')
+    if issubclass(base, rhf.DeviceRHF):
+        # the device-resident loop takes the fit itself (mjk_rhf_set_df)
+        new = base.__new__(base)
+        new.__dict__.update(mf.__dict__)
+        new.with_df = with_df or DF(mf.mol, auxbasis)
+        return new
     new = DFHF.__new__(DFHF)
     new.__dict__.update(mf.__dict__)
     new.with_df = with_df or DF(mf.mol, auxbasis)

@@ -89,6 +89,7 @@ def _declare(lib):
     lib.mjk_rhf_run.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_double, c_double, c_double, c_double,
                                 c_double, P(mjk_rhf_result)]
     lib.mjk_rhf_fetch.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.mjk_rhf_set_df.argtypes = [c_void_p, c_void_p, c_void_p]
     lib.mjk_rhf_device_density.argtypes = [c_void_p]
     lib.mjk_rhf_device_density.restype = c_void_p
     lib.mjk_rhf_destroy.argtypes = [c_void_p]

@@ -235,7 +235,9 @@ class DeviceRHF(RHF):
     the dense algebra runs on rocBLAS/rocSOLVER, J/K on the engine.  Same
     inputs, attributes and convergence test as RHF; the host sees the DIIS
     overlaps and three scalars per cycle.  The initial guess is the core
-    Hamiltonian, or dm0."""
+    Hamiltonian, or dm0.  With ``with_df`` (``density_fit()``) every cycle
+    builds density-fitted J/K of the full density, K from the occupied
+    orbitals, all on the device."""
 
     def kernel(self, dm0=None):
         mol = self.mol
@@ -258,6 +260,14 @@ class DeviceRHF(RHF):
         eng.check(L.mjk_rhf_create(eng.handle, opt.basis.handle, p(s1e), p(h1e), mol.nelectron // 2, enuc,
                                    self.diis_space, ctypes.byref(h)), 'mjk_rhf_create')
         try:
+            with_df = getattr(self, 'with_df', None)
+            if with_df is not None:
+                if with_df.engine is None:
+                    with_df.engine = eng
+                if with_df.engine is not eng:
+                    raise ValueError('the density fit belongs to another engine')
+                with_df.build()
+                eng.check(L.mjk_rhf_set_df(eng.handle, h, with_df.handle), 'mjk_rhf_set_df')
             if dm0 is not None:
                 dm0 = numpy.ascontiguousarray(dm0, dtype=numpy.double)
             res = _lib.mjk_rhf_result()

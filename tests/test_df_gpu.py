@@ -108,3 +108,24 @@ def test_cderi_packed_layout():
     i, j = numpy.tril_indices(24)
     assert packed.shape == (d.get_naoaux(), 24 * 25 // 2)
     assert abs(packed - b[:, i, j]).max() == 0 and abs(b - b.transpose(0, 2, 1)).max() <= 1e-13 * abs(b).max()
+
+
+def test_device_resident_df_rhf():
+    """The density-fitted RHF with the whole loop on the device (mjk_rhf_set_df)
+    against the host-driven density-fitted RHF and the reference's golden
+    energy."""
+    mol = gto.M(atom=H2O, basis='cc-pvdz')
+    dev = rhf.DeviceRHF(mol).density_fit(auxbasis='weigend')
+    dev.conv_tol = 1e-11
+    e = dev.kernel()
+    print('device DF-RHF E=%.12f (%d cycles)' % (e, dev.cycles), flush=True)
+    assert dev.converged and abs(e - -76.025936299702536) < 1e-8
+    mol = CASES['chain_vdz_jkfit'][0]()
+    host = rhf.RHF(mol).density_fit()
+    host.conv_tol = 1e-11
+    eh = host.kernel()
+    dev = rhf.DeviceRHF(mol).density_fit()
+    dev.conv_tol = 1e-11
+    ed = dev.kernel()
+    print('chain DF-RHF host %.12f device %.12f' % (eh, ed), flush=True)
+    assert host.converged and dev.converged and abs(eh - ed) < 1e-9
