@@ -83,7 +83,8 @@ def traffic_bytes(kernel, workload):
     path = os.path.join(ROOT, 'profiles', 'jk_%s_traffic.json' % workload)
     try:
         with open(path) as f:
-            rec = json.load(f)['kernels'][kernel]
+            kernels = json.load(f)['kernels']
+        rec = kernels[kernel] if kernel in kernels else kernels['void ' + kernel]   # the trace prints 'void '
     except (OSError, KeyError, ValueError):
         return None
     return rec['fetch_bytes_per_launch'] + rec['write_bytes_per_launch']
