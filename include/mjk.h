@@ -78,8 +78,8 @@ size_t mjk_ctx_hbm_used(const mjk_ctx* ctx);
  * rows in LDS).  Classes whose rows do not fit run their fallback kernel
  * (the tile kernel up to 81 Cartesian components per quartet, the
  * cooperative kernel above); 0 runs the fallback kernels only.  Default
- * 98304; at most 161792 (a gfx950 workgroup's 160 KB less 2 KB).  Results
- * agree either way to rounding. */
+ * 98304; at most 147456 (a gfx950 workgroup's 160 KB less 16 KB for the
+ * kernels' static LDS).  Results agree either way to rounding. */
 int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes);
 
 /* No reference counterpart (a launch-overhead option): a build without

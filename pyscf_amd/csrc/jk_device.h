@@ -207,6 +207,25 @@ struct QuartetCtx {
     int pb0, npb, pk0, npk;
     int tq, ts;          // ket primitives in the transposed copy: tP[tq + ik * ts]
     double fac;
+    const double* rtab;  // the block's LDS copy of the Rys table section (row kernels)
+};
+
+// Rys table of the row kernels: classes of at most two roots (la + lb + lc +
+// ld <= 3) read the table from a copy in LDS that each block makes once
+// (11 KB for two roots).  Their integrals are light and their root series
+// are most of their memory reads: from the vector cache, the 16-byte gathers
+// of 64 lanes kept the texture address and data units 80-90% busy (MI355X,
+// C30H62 cc-pVDZ, rocprofv3 TA_TA_BUSY / TD_TD_BUSY), and from LDS the build
+// runs 1.18x faster.  Classes of three or more roots gained nothing overall
+// (the larger copy costs LDS occupancy and fill time) and read the table in
+// memory.  (ss|ss) needs no table.
+template <int LA, int LB, int LC, int LD>
+struct RowRys {
+    static constexpr int L = LA + LB + LC + LD;
+    static constexpr int nr = L / 2 + 1;
+    static constexpr bool lds = nr <= 2 && L > 0;
+    static constexpr int size = lds ? RysLds<nr>::SIZE : 0;
+    static_assert(size * 8 <= 12 * 1024, "static LDS of the row kernels stays within the 16 KB the host leaves");
 };
 
 // Sum over the 8 lanes sharing the bra pair (xor 1,2,4) or the ket pair
@@ -314,13 +333,16 @@ __device__ __forceinline__ void add_root(const PrimQuartet& m, double rx, double
 // Contracted Cartesian block of one quartet restricted to slice S, times
 // the quartet's degeneracy factor, in FP64 (recurrences in precision R).
 // RSH: the kernel also evaluates the range-separated operators (a.omega != 0).
-template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R, bool UNIFORM_BRA = false, bool RSH = false>
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R, bool UNIFORM_BRA = false, bool RSH = false,
+          bool LT = false>
 __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
                                           double (&gd)[SliceRange<LA, LB, LC, LD, SPLIT, S>::NFL])
 {
     using SR = SliceRange<LA, LB, LC, LD, SPLIT, S>;
     constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
     constexpr int NFL = SR::NFL;
+    using Rys = RysFor<NR, LA + LB + LC + LD, LT>;
+    const double* const rt = LT ? c.rtab : a.rys;
 
     R gout[NFL];
 #pragma unroll
@@ -338,7 +360,7 @@ __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
                 for (int pass = 0; pass < rsh_passes(a.omega); ++pass) {
                     double Tp = T, sx, sw;
                     rsh_scale(a.omega, rho, pass, Tp, sx, sw);
-                    const typename RysFor<NR, LA + LB + LC + LD>::type rys(a.rys, Tp);
+                    const typename Rys::type rys(rt, Tp);
 #pragma unroll 1
                     for (int r = 0; r < NR; ++r) {
                         double rx, rw;
@@ -356,7 +378,7 @@ __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
                 const int t0 = c.tq + ik * c.ts, t1 = t0 + c.ts;
                 const PrimQuartet m0 = prim_quartet(Pp, Kp, a.P.tP[t0], a.P.tK[t0], c, T0);
                 const PrimQuartet m1 = prim_quartet(Pp, Kp, a.P.tP[t1], a.P.tK[t1], c, T1);
-                const typename RysFor<NR, LA + LB + LC + LD>::type rys0(a.rys, T0), rys1(a.rys, T1);
+                const typename Rys::type rys0(rt, T0), rys1(rt, T1);
 #pragma unroll 1
                 for (int r = 0; r < NR; ++r) {
                     double rx0, rw0, rx1, rw1;
@@ -371,7 +393,7 @@ __device__ __forceinline__ void eri_slice(const JKArgs& a, const QuartetCtx& c,
             double T;
             const int t0 = c.tq + ik * c.ts;
             const PrimQuartet m = prim_quartet(Pp, Kp, a.P.tP[t0], a.P.tK[t0], c, T);
-            const typename RysFor<NR, LA + LB + LC + LD>::type rys(a.rys, T);
+            const typename Rys::type rys(rt, T);
 #pragma unroll 1
             for (int r = 0; r < NR; ++r) {
                 double rx, rw;
@@ -934,7 +956,7 @@ __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c,
     constexpr int IA0 = SR::IA0, IA1 = SR::IA1, IB0 = SR::IB0, IB1 = SR::IB1, IC0 = SR::IC0, IC1 = SR::IC1;
     constexpr int MA = SR::MA, MB = SR::MB, MC = SR::MC;
     double gd[SR::NFL];
-    eri_slice<LA, LB, LC, LD, SPLIT, S, R, true, RSH>(a, c, gd);
+    eri_slice<LA, LB, LC, LD, SPLIT, S, R, true, RSH, RowRys<LA, LB, LC, LD>::lds>(a, c, gd);
     const int oc = c.qvalid ? a.P.cartA[c.q] : 0, od = c.qvalid ? a.P.cartB[c.q] : 0;
     const int lane = threadIdx.x & 63;
     for (int s = 0; s < a.n_j; ++s) {
@@ -1076,6 +1098,12 @@ void k_jk_row(const JKArgs a)
     __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
     __shared__ int s_next;   // next unclaimed ket of the chunk (waves take 64 at a time)
     const int tid = threadIdx.x, lane = tid & 63;
+    using RT = RowRys<LA, LB, LC, LD>;
+    __shared__ double s_rys[RT::size > 0 ? RT::size : 1];
+    if constexpr (RT::size > 0) {
+        rys_lds_fill<RT::nr>(s_rys, a.rys, tid, 256);
+        __syncthreads();
+    }
     __shared__ int s_task;
     unsigned long long cnt[4] = {0, 0, 0, 0};
     const bool persist = a.thead != nullptr;
@@ -1125,6 +1153,7 @@ void k_jk_row(const JKArgs a)
     c.p = p;
     c.mol = mol;
     c.pvalid = true;
+    c.rtab = s_rys;
     c.rA = uload(a.P.rA + p);
     c.AB = uload(a.P.rAB + p);
     const int pbo = uload(a.rtp + tb);   // this task's share of the bra's primitive pairs
@@ -1295,7 +1324,8 @@ __device__ __forceinline__ void eri_gc(const JKArgs& a, const QuartetCtx& c,
             R tmp[NF];
 #pragma unroll
             for (int f = 0; f < NF; ++f) tmp[f] = R(0);
-            const typename RysFor<NR, LA + LB + LC + LD>::type rys(a.rys, T);
+            using Rys = RysFor<NR, LA + LB + LC + LD, RowRys<LA, LB, LC, LD>::lds>;
+            const typename Rys::type rys(Rys::lds > 0 ? c.rtab : a.rys, T);
 #pragma unroll 1
             for (int r = 0; r < NR; ++r) {
                 double rx, rw;
@@ -1510,6 +1540,12 @@ __global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
     __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
     __shared__ int s_next;
     const int tid = threadIdx.x, lane = tid & 63;
+    using RT = RowRys<LA, LB, LC, LD>;
+    __shared__ double s_rys[RT::size > 0 ? RT::size : 1];
+    if constexpr (RT::size > 0) {
+        rys_lds_fill<RT::nr>(s_rys, a.rys, tid, 256);
+        __syncthreads();
+    }
     __shared__ int s_task;
     unsigned long long cnt[4] = {0, 0, 0, 0};
     const bool persist = a.thead != nullptr;
@@ -1557,6 +1593,7 @@ __global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
     c.p = p;
     c.mol = mol;
     c.pvalid = true;
+    c.rtab = s_rys;
     c.rA = uload(a.P.rA + p);
     c.AB = uload(a.P.rAB + p);
     const int pbo = uload(a.rtp + tb);

@@ -77,8 +77,9 @@ namespace {
 // cutoff an override may set (MJK_EXPCUTOFF) is libcint's 60.
 constexpr double EXPCUTOFF_DEFAULT = 40.0;
 // Dynamic LDS a row-kernel block may request: the 160 KB of a gfx950 workgroup
-// less room for the kernel's static shared variables.
-constexpr int ROW_LDS_MAX = 160 * 1024 - 2048;
+// less 16 KB for the kernel's static shared variables (its copy of the Rys
+// table, at most 12 KB, and the task bookkeeping).
+constexpr int ROW_LDS_MAX = 160 * 1024 - 16 * 1024;
 constexpr double EXPCUTOFF_MAX = 60.0;
 constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8, PTR_COORD = 1, ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2,
               NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6, PTR_RANGE_OMEGA = 8;

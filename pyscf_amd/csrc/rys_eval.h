@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #define RYS_STORAGE static constexpr
 #include "rys_table.inc"
 #undef RYS_STORAGE
@@ -113,6 +115,68 @@ struct RysCursor {
     }
 };
 
+// The n-root section of the table copied into LDS by a block (kernels whose
+// roots come from one section): every interval's 2n series followed by the
+// Hermite constants.  The interval stride is padded from 28n doubles (a
+// multiple of 4) to 28n + 2: intervals then start 4 LDS banks apart modulo
+// 32, so lanes of one wave reading the same coefficient of different
+// intervals spread over 8 bank groups of their 16-byte reads.
+template <int N>
+struct RysLds {
+    static constexpr int NINT = 18 + 3 * N;          // intervals of T < 36 + 6n
+    static constexpr int SER = N * 2 * RYS_NCOEF;    // doubles of one interval
+    static constexpr int STRIDE = SER + 2;
+    static constexpr int SIZE = NINT * STRIDE + 2 * N;
+};
+
+// Copy of the n-root section into lt (RysLds<N>::SIZE doubles) by nthr
+// threads; the caller synchronises the block before the first cursor reads it.
+template <int N>
+__device__ __forceinline__ void rys_lds_fill(double* lt, const double* __restrict__ tab, int tid, int nthr)
+{
+    using L = RysLds<N>;
+    const double* src = tab + RYS_OFF[N];
+    for (int e = tid; e < L::NINT * L::SER; e += nthr) {
+        const int iv = e / L::SER;
+        lt[iv * L::STRIDE + (e - iv * L::SER)] = src[e];
+    }
+    if (tid < 2 * N) lt[L::NINT * L::STRIDE + tid] = tab[RYS_HERM_BASE + RYS_HOFF[N] + tid];
+}
+
+// RysCursor over the LDS copy.
+template <int N>
+struct RysCursorLds {
+    const double* c;
+    double s, st;
+    bool asym;
+    __device__ __forceinline__ RysCursorLds(const double* lt, double T)
+    {
+        using L = RysLds<N>;
+        asym = !(T < 36.0 + 6.0 * N);
+        if (asym) {
+            st = rsqrt_nr(T);
+            s = st * st;
+            c = lt + L::NINT * L::STRIDE;
+        } else {
+            int iv = (int)(T * 0.5);
+            iv = iv < 0 ? 0 : (iv > L::NINT - 1 ? L::NINT - 1 : iv);
+            s = T - 2.0 * iv - 1.0;
+            st = 0.0;
+            c = lt + iv * L::STRIDE;
+        }
+    }
+    __device__ __forceinline__ void root(int i, double& x, double& w) const
+    {
+        if (asym) {
+            x = c[2 * i] * s;
+            w = c[2 * i + 1] * st;
+        } else {
+            x = poly14(c + (2 * i) * RYS_NCOEF, s);
+            w = poly14(c + (2 * i + 1) * RYS_NCOEF, s);
+        }
+    }
+};
+
 // Range-separated operators, selected by omega (env[PTR_RANGE_OMEGA] of the
 // basis): 0 the Coulomb 1/r, > 0 erf(omega r)/r, < 0 erfc(|omega| r)/r.  The
 // erf kernel restricts the Rys variable to t^2 <= theta = omega^2 / (omega^2 +
@@ -166,14 +230,15 @@ struct RysWeightOnly {
     }
 };
 
-// Root evaluator of a class: weight only when L = la + lb + lc + ld = 0.
-template <int NR, int L>
+// Root evaluator of a class: weight only when L = la + lb + lc + ld = 0;
+// LDS: the cursor reads the block's LDS copy of the section (lds doubles of
+// it, 0 when the class needs no table).
+template <int NR, int L, bool LDS = false>
 struct RysFor {
-    using type = RysCursor<NR>;
-};
-template <>
-struct RysFor<1, 0> {
-    using type = RysWeightOnly;
+    static constexpr bool weight_only = NR == 1 && L == 0;
+    using type = typename std::conditional<weight_only, RysWeightOnly,
+                                           typename std::conditional<LDS, RysCursorLds<NR>, RysCursor<NR>>::type>::type;
+    static constexpr int lds = (LDS && !weight_only) ? RysLds<NR>::SIZE : 0;
 };
 
 // Host-side helper: the table as one flat array (series then Hermite
