@@ -49,6 +49,7 @@ extern "C" {
 #define MJK_ENOMEM (-2)       /* request exceeds the context's HBM budget */
 #define MJK_EDEVICE (-3)      /* HIP runtime or kernel failure */
 #define MJK_EUNSUPPORTED (-4) /* angular momentum or feature not built */
+#define MJK_EINTERNAL (-5)    /* internal consistency check failed */
 
 typedef struct mjk_ctx mjk_ctx;
 typedef struct mjk_basis mjk_basis;
@@ -422,7 +423,9 @@ int mjk_int1e(int which, const int* atm, int natm, const int* bas, int nbas, con
 int mjk_c2s(int l, double* out);
 
 /* Rys roots x_i (= t_i^2) and weights of the interpolation table, evaluated
- * on the host (table validation).  n <= 9.  No reference counterpart (the
+ * on the host (table validation).  n <= 9.  The kernels' root cursor is also
+ * run over the table in memory and over the padded per-root-count layout that
+ * the row kernels copy to LDS; MJK_EINTERNAL if the two differ in any bit.  No reference counterpart (the
  * reference's roots live inside libcint). */
 int mjk_rys_roots(int n, double T, double* x, double* w);
 

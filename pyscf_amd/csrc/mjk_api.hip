@@ -2326,16 +2326,27 @@ int mjk_rys_roots(int n, double T, double* x, double* w)
         tab.resize(RYS_BUF_SIZE);
         rys_fill_buffer(tab.data());
     }
+    // the kernels' cursor over the table in memory and over the padded layout
+    // the row kernels copy to LDS must agree bit for bit (same coefficients,
+    // same evaluation order)
     switch (n) {
-#define R(N)                                          \
-    case N: {                                         \
-        double xx[N], ww[N];                          \
-        rys_roots<N>(tab.data(), T, xx, ww);          \
-        for (int i = 0; i < N; ++i) {                 \
-            x[i] = xx[i];                             \
-            w[i] = ww[i];                             \
-        }                                             \
-        return MJK_OK;                                \
+#define R(N)                                                          \
+    case N: {                                                         \
+        double xx[N], ww[N];                                          \
+        rys_roots<N>(tab.data(), T, xx, ww);                          \
+        std::vector<double> lt(RysLds<N>::SIZE);                      \
+        rys_lds_fill<N>(lt.data(), tab.data(), 0, 1);                 \
+        const RysCursorLds<N> cur(lt.data(), T);                      \
+        const RysCursor<N> ref(tab.data(), T);                        \
+        for (int i = 0; i < N; ++i) {                                 \
+            double xl, wl, xg, wg;                                    \
+            cur.root(i, xl, wl);                                      \
+            ref.root(i, xg, wg);                                      \
+            if (xl != xg || wl != wg) return MJK_EINTERNAL;           \
+            x[i] = xx[i];                                             \
+            w[i] = ww[i];                                             \
+        }                                                             \
+        return MJK_OK;                                                \
     }
         R(1) R(2) R(3) R(4) R(5) R(6) R(7) R(8) R(9)
 #undef R

@@ -132,7 +132,7 @@ struct RysLds {
 // Copy of the n-root section into lt (RysLds<N>::SIZE doubles) by nthr
 // threads; the caller synchronises the block before the first cursor reads it.
 template <int N>
-__device__ __forceinline__ void rys_lds_fill(double* lt, const double* __restrict__ tab, int tid, int nthr)
+__host__ __device__ __forceinline__ void rys_lds_fill(double* lt, const double* __restrict__ tab, int tid, int nthr)
 {
     using L = RysLds<N>;
     const double* src = tab + RYS_OFF[N];
@@ -140,16 +140,16 @@ __device__ __forceinline__ void rys_lds_fill(double* lt, const double* __restric
         const int iv = e / L::SER;
         lt[iv * L::STRIDE + (e - iv * L::SER)] = src[e];
     }
-    if (tid < 2 * N) lt[L::NINT * L::STRIDE + tid] = tab[RYS_HERM_BASE + RYS_HOFF[N] + tid];
+    for (int e = tid; e < 2 * N; e += nthr) lt[L::NINT * L::STRIDE + e] = tab[RYS_HERM_BASE + RYS_HOFF[N] + e];
 }
 
-// RysCursor over the LDS copy.
+// RysCursor over the LDS copy (or a host buffer of the same layout).
 template <int N>
 struct RysCursorLds {
     const double* c;
     double s, st;
     bool asym;
-    __device__ __forceinline__ RysCursorLds(const double* lt, double T)
+    __host__ __device__ __forceinline__ RysCursorLds(const double* lt, double T)
     {
         using L = RysLds<N>;
         asym = !(T < 36.0 + 6.0 * N);
@@ -165,7 +165,7 @@ struct RysCursorLds {
             c = lt + iv * L::STRIDE;
         }
     }
-    __device__ __forceinline__ void root(int i, double& x, double& w) const
+    __host__ __device__ __forceinline__ void root(int i, double& x, double& w) const
     {
         if (asym) {
             x = c[2 * i] * s;
