@@ -51,6 +51,9 @@ struct mjk_df {
     size_t caprho = 0;
     void* ws = nullptr;
     size_t capws = 0;
+    // blocks of the exchange product from the occupied orbitals (0: by size;
+    // MJK_DF_KBLOCKS overrides)
+    int kblocks = 0;
 };
 
 namespace {
@@ -505,6 +508,102 @@ int build(mjk_ctx* ctx, mjk_df* df, const int* atm, int natm, const int* bas, in
     return MJK_OK;
 }
 
+// rocBLAS's DGEMM for the long-inner-dimension shapes of the exchange
+// product (m, n = nao, inner dimension naux nocc or naux nao) runs at about
+// 50 TFLOP/s for even m and n and at a fraction of that for odd ones
+// (measured on MI355X, inner dimension 426646 and a 512 MiB workspace: m = n
+// = 730 in 9.1 ms, 729 and 731 in 68.7 ms; every odd m in 183..731 tried took
+// 68.4-68.8 ms, every even one 0.9-9.1 ms).  So the GEMMs below see even
+// dimensions only; an odd last AO is a matrix-vector product.
+
+// Block of an index in the lower-block layout of df_syrk: blocks of bs over
+// [0, ne), the odd last index (ne = nao - 1) a block of its own after them.
+__device__ __forceinline__ int df_blk(int x, int ne, int bs) { return x >= ne ? 1 << 30 : x / bs; }
+
+// Upper part of a symmetric nao x nao matrix (column-major view) whose lower
+// blocks (block of the row >= block of the column) are computed.
+__global__ void k_df_mirror(double* K, int nao, int ne, int bs)
+{
+    const long long n = (long long)nao * nao;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(e / nao), r = (int)(e - (long long)c * nao);
+        if (df_blk(r, ne, bs) < df_blk(c, ne, bs)) K[e] = K[(long long)r * nao + c];
+    }
+}
+
+// Inside a workspace-size query rocBLAS calls only report sizes (statuses
+// other than success): failures count outside a query only.
+#define RBQ(ctx, query, x)                                                                            \
+    do {                                                                                              \
+        rocblas_status s_ = (x);                                                                      \
+        if (!(query) && s_ != rocblas_status_success)                                                 \
+            return ctx_fail(ctx, MJK_EDEVICE, "%s failed: %s", #x, rocblas_status_to_string(s_));     \
+    } while (0)
+
+// Blocks of the symmetric exchange product for nao AOs: one block (a single
+// GEMM) below 640 AOs, else about 250 AOs per block, a width at which the
+// GEMMs keep the full product's rate (49-57 TFLOP/s for 242-256 and 486-512
+// rows, 30 for 184, 15 for 122) while the lower triangle needs (1 + 1/nb) / 2
+// of its FLOPs.  C30H62 cc-pVDZ / cc-pVDZ-JKFIT (730 AOs, blocks 244, 244,
+// 242): the build from the occupied orbitals 21.1 -> 17.8 ms.  kblocks > 0
+// overrides.
+int df_kblocks(int nao, int kblocks)
+{
+    if (kblocks > 0) return kblocks;
+    return nao < 640 ? 1 : (nao + 125) / 250;
+}
+
+// K = Y^T Y (column-major nao x nao; Y: kin x nao, column-major) as the
+// lower blocks of nb x nb even-sized blocks over the even part of the AOs, the
+// odd last AO's row by DGEMV, then the mirror.
+int df_syrk(mjk_ctx* ctx, rocblas_handle h, int nb, int nao, int kin, const double* Y, double* K, hipStream_t st)
+{
+    const double one = 1.0, zero = 0.0;
+    const bool query = rocblas_is_device_memory_size_query(h);
+    const int ne = nao & ~1;
+    nb = std::max(1, std::min(nb, ne / 2));
+    int bs = (ne + nb - 1) / nb;
+    bs += bs & 1;   // even blocks (the last one too: ne is even)
+    for (int i0 = 0; i0 < ne; i0 += bs) {
+        const int ni = std::min(bs, ne - i0);
+        for (int j0 = 0; j0 <= i0; j0 += bs) {
+            const int nj = std::min(bs, ne - j0);
+            RBQ(ctx, query, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ni, nj, kin, &one,
+                                          Y + (size_t)i0 * kin, kin, Y + (size_t)j0 * kin, kin, &zero,
+                                          K + (size_t)j0 * nao + i0, nao));
+        }
+    }
+    if (ne < nao)   // row nao - 1: K[nao-1, c] = y_{nao-1} . y_c, all c
+        RBQ(ctx, query, rocblas_dgemv(h, rocblas_operation_transpose, kin, nao, &one, Y, kin, Y + (size_t)ne * kin, 1,
+                                      &zero, K + ne, nao));
+    if (query || (bs >= ne && ne == nao)) return MJK_OK;
+    k_df_mirror<<<1024, 256, 0, st>>>(K, nao, ne, bs);
+    HC(ctx, hipGetLastError());
+    return MJK_OK;
+}
+
+// K = B^T Y (column-major nao x nao; B, Y: kin x nao, column-major), even
+// GEMM dimensions: the ne x ne block by DGEMM, an odd last AO's column and
+// row by DGEMV.
+int df_gemm_tn(mjk_ctx* ctx, rocblas_handle h, int nao, int kin, const double* B, const double* Y, double* K)
+{
+    const double one = 1.0, zero = 0.0;
+    const bool query = rocblas_is_device_memory_size_query(h);
+    const int ne = nao & ~1;
+    if (ne > 0)
+        RBQ(ctx, query, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, ne, ne, kin, &one, B,
+                                      kin, Y, kin, &zero, K, nao));
+    if (ne < nao) {
+        // column nao - 1: K[r, nao-1] = b_r . y_{nao-1}, all r
+        RBQ(ctx, query, rocblas_dgemv(h, rocblas_operation_transpose, kin, nao, &one, B, kin, Y + (size_t)ne * kin, 1,
+                                      &zero, K + (size_t)ne * nao, 1));
+        // row nao - 1: K[nao-1, c] = b_{nao-1} . y_c, c < nao - 1
+        RBQ(ctx, query, rocblas_dgemv(h, rocblas_operation_transpose, kin, ne, &one, Y, kin, B + (size_t)ne * kin, 1,
+                                      &zero, K + ne, nao));
+    }
+    return MJK_OK;
+}
+
 // J and K of n_dm densities (device, nao x nao each, row-major), optional
 // occupied orbitals X (n_dm x nao x nocc, row-major) for K
 int get_jk_dev(mjk_df* df, const double* d_dms, int n_dm, const double* d_orbo, int nocc, int with_j, int with_k,
@@ -529,8 +628,10 @@ int get_jk_dev(mjk_df* df, const double* d_dms, int n_dm, const double* d_orbo, 
             (void)rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, naux, ncol, nao,
                                                 &one, df->B, naux, (rocblas_stride)naux * nao, d_dms, ncol, 0, &zero,
                                                 df->Y, naux, (rocblas_stride)naux * ncol, nao);
-            (void)rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nao, nao, naux * ncol, &one,
-                                df->Y, naux * ncol, df->Y, naux * ncol, &zero, d_vk, nao);
+            if (d_orbo)
+                (void)df_syrk(ctx, h, df_kblocks(nao, df->kblocks), nao, naux * ncol, df->Y, d_vk, st);
+            else
+                (void)df_gemm_tn(ctx, h, nao, naux * ncol, df->B, df->Y, d_vk);
         }
         RB(ctx, rocblas_stop_device_memory_size_query(h, &need));
         if ((rc = workspace(df, h, need))) return rc;
@@ -568,13 +669,11 @@ int get_jk_dev(mjk_df* df, const double* d_dms, int n_dm, const double* d_orbo, 
                                                   cm ? nao : ncol, 0, &zero, df->Y, naux, (rocblas_stride)naux * ncol,
                                                   nao));
             if (d_orbo) {
-                // K = Y^T Y over (P, o)
-                RB(ctx, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nao, nao, naux * ncol,
-                                      &one, df->Y, naux * ncol, df->Y, naux * ncol, &zero, K, nao));
+                // K = Y^T Y over (P, o), symmetric
+                if ((rc = df_syrk(ctx, h, df_kblocks(nao, df->kblocks), nao, naux * ncol, df->Y, K, st))) return rc;
             } else {
                 // K (row-major) = B^T Y over (P, l): column-major C(k, i) = K[i][k]
-                RB(ctx, rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, nao, nao, naux * nao,
-                                      &one, df->B, naux * nao, df->Y, naux * nao, &zero, K, nao));
+                if ((rc = df_gemm_tn(ctx, h, nao, naux * nao, df->B, df->Y, K))) return rc;
             }
         }
     }
@@ -632,6 +731,7 @@ int mjk_df_build(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nba
     if (env[8] != 0.0) return ctx_fail(ctx, MJK_EUNSUPPORTED, "mjk_df_build: Coulomb operator only (omega = 0)");
     mjk_df* df = new mjk_df();
     df->ctx = ctx;
+    if (const char* kb = getenv("MJK_DF_KBLOCKS")) df->kblocks = std::max(1, std::min(16, atoi(kb)));
     const int rc = build(ctx, df, atm, natm, bas, nbas, env, aux_atm, aux_natm, aux_bas, aux_nbas, aux_env, lindep);
     if (rc) {
         release(df);

@@ -11,6 +11,8 @@ reference's golden values.
 * Golden values of pyscf/df/test/test_df_jk.py: lib.fp(vj), lib.fp(vk) of
   the seed-1 densities (9 places) and the DF-RHF energy (8 places).
 * K from occupied orbitals (the SCF path) equals K of their density.
+* The exchange GEMMs on even dimensions and in lower blocks (odd AO count,
+  forced blocks) against the oracle.
 """
 import numpy
 import pytest
@@ -74,6 +76,38 @@ def test_fit_and_jk_vs_oracle(oracle, name):
         j0, k0 = oracle.df_get_jk(cd, dms, hermi)
         print('  hermi=%d max|dJ|=%.2e max|dK|=%.2e' % (hermi, abs(vj - j0).max(), abs(vk - k0).max()), flush=True)
         assert abs(vj - j0).max() <= TOL and abs(vk - k0).max() <= TOL
+
+
+@pytest.mark.parametrize('kblocks', ['0', '3'])
+def test_exchange_even_gemms_and_blocks(oracle, monkeypatch, kblocks):
+    """The exchange products run their GEMMs on an even number of AOs (an odd
+    last AO by matrix-vector products) and, with more than one block, as the
+    lower blocks of the symmetric product, mirrored: NH3 cc-pVDZ (29 AOs) and
+    H2O cc-pVDZ (24 AOs), general and occupied-orbital paths, against the
+    oracle's fit (kblocks 0 keeps one block at these sizes, 3 forces three)."""
+    monkeypatch.setenv('MJK_DF_KBLOCKS', kblocks)   # read when a DF object is built
+    nh3 = 'N 0 0 0.11; H 0 0.94 -0.26; H 0.81 -0.47 -0.26; H -0.81 -0.47 -0.26'
+    for atom in (nh3, H2O):
+        mol = gto.M(atom=atom, basis='cc-pvdz')
+        nao = mol.nao_nr()
+        d = df.DF(mol, 'weigend').build()
+        cd = oracle.cholesky_eri(mol, d.auxmol)
+        for hermi, sym in ((0, False), (1, True)):
+            dms = dens(nao, 11 + hermi, sym=sym)
+            vj, vk = df.get_jk(d, dms, hermi)
+            j0, k0 = oracle.df_get_jk(cd, dms, hermi)
+            print(nao, 'kblocks', kblocks, 'hermi=%d max|dJ|=%.2e max|dK|=%.2e'
+                  % (hermi, abs(vj - j0).max(), abs(vk - k0).max()), flush=True)
+            assert abs(vj - j0).max() <= TOL and abs(vk - k0).max() <= TOL
+        rng = numpy.random.default_rng(3)
+        c = rng.standard_normal((nao, nao)) / nao
+        occ = numpy.zeros(nao)
+        occ[:5] = 2.0
+        dm = 2.0 * c[:, :5] @ c[:, :5].T
+        vj, vk = df.get_jk(d, lib.tag_array(dm, mo_coeff=c, mo_occ=occ), 1)
+        j0, k0 = oracle.df_get_jk(cd, dm[None], 1)
+        print(nao, 'kblocks', kblocks, 'orbitals max|dK|=%.2e' % abs(vk - k0[0]).max(), flush=True)
+        assert abs(vj - j0[0]).max() <= TOL and abs(vk - k0[0]).max() <= TOL
 
 
 def test_golden_fingerprints():
