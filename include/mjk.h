@@ -76,11 +76,12 @@ size_t mjk_ctx_hbm_used(const mjk_ctx* ctx);
 /* Kernel selection knob (no counterpart in the reference, whose loop has a
  * single CPU code path): the largest LDS allocation, in bytes, that a
  * row-kernel block may take (one block per bra pair accumulating its K'
- * rows in LDS).  Classes whose rows do not fit run their fallback kernel
- * (the tile kernel up to 81 Cartesian components per quartet, the
- * cooperative kernel above); 0 runs the fallback kernels only.  Default
- * 98304; at most 147456 (a gfx950 workgroup's 160 KB less 16 KB for the
- * kernels' static LDS).  Results agree either way to rounding. */
+ * rows in LDS).  Classes whose rows do not fit this limit, or with the
+ * kernel's static LDS (up to 12 KB: the Rys table copy of the one- and
+ * two-root classes) the workgroup's 160 KB, run their fallback kernel (the
+ * tile kernel up to 81 Cartesian components per quartet, the cooperative
+ * kernel above); 0 runs the fallback kernels only.  Default and maximum
+ * 162816 (160 KB less 1 KB).  Results agree either way to rounding. */
 int mjk_ctx_set_row_lds(mjk_ctx* ctx, size_t bytes);
 
 /* No reference counterpart (a launch-overhead option): a build without

@@ -76,10 +76,11 @@ namespace {
 // summation-order noise of two builds); e^-30 moved J by 8e-10.  The largest
 // cutoff an override may set (MJK_EXPCUTOFF) is libcint's 60.
 constexpr double EXPCUTOFF_DEFAULT = 40.0;
-// Dynamic LDS a row-kernel block may request: the 160 KB of a gfx950 workgroup
-// less 16 KB for the kernel's static shared variables (its copy of the Rys
-// table, at most 12 KB, and the task bookkeeping).
-constexpr int ROW_LDS_MAX = 160 * 1024 - 16 * 1024;
+// LDS of a gfx950 workgroup, and the most a row-kernel block's dynamic rows
+// may take (less 1 KB for its task bookkeeping); a class's rows must also
+// leave room for its static copy of the Rys table (row_static_lds).
+constexpr int LDS_WORKGROUP = 160 * 1024;
+constexpr int ROW_LDS_MAX = LDS_WORKGROUP - 1024;
 constexpr double EXPCUTOFF_MAX = 60.0;
 constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8, PTR_COORD = 1, ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2,
               NCTR_OF = 3, PTR_EXP = 5, PTR_COEFF = 6, PTR_RANGE_OMEGA = 8;
@@ -196,7 +197,7 @@ struct mjk_ctx {
     // LDS bytes a row-kernel block may take; 0 = fallback kernels only.  gfx950
     // gives a workgroup up to 160 KB; 96 KB keeps room for a second block per CU
     // and let (dd|ss) on C30H62 cc-pVDZ use rows: 73.0 -> 68.9 ms per build
-    size_t row_lds = 98304;
+    size_t row_lds = ROW_LDS_MAX;
     int row_chunk = 16384;    // least primitive quartets per row-kernel block
     int row_blocks = 16384;   // row-kernel blocks per class launch to aim for ...
     int row_min_blocks = 512;    // ... and the fewest, whatever row_chunk asks
@@ -840,6 +841,26 @@ size_t row_shmem(const QClass& q, int n_j, int n_k, int maxld)
     return sizeof(double) * ((size_t)n_k * (ra + rb) * maxld + (size_t)n_j * ra * rb);
 }
 
+// Static LDS of a row-kernel block of class q beside its dynamic rows: the
+// copy of the Rys table that the one- and two-root classes read (the rule of
+// RowRys in jk_device.h) and 1 KB for the task bookkeeping.
+size_t row_static_lds(const QClass& q)
+{
+    const int L = q.la + q.lb + q.lc + q.ld, nr = L / 2 + 1;
+    size_t t = 0;
+    if (L > 0 && nr == 1) t = sizeof(double) * RysLds<1>::SIZE;
+    if (L > 0 && nr == 2) t = sizeof(double) * RysLds<2>::SIZE;
+    return t + 1024;
+}
+
+// Whether class q runs its row kernel as far as LDS goes: its rows within the
+// context's limit, rows and static LDS within the workgroup's.
+bool row_fits(const mjk_ctx* ctx, const QClass& q, int n_j, int n_k, int maxld)
+{
+    const size_t sh = row_shmem(q, n_j, n_k, maxld);
+    return sh <= ctx->row_lds && sh + row_static_lds(q) <= (size_t)LDS_WORKGROUP;
+}
+
 int parse_molecule(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas, const double* env,
                    MolHost& m)
 {
@@ -1291,7 +1312,7 @@ std::vector<char> classes_to_run(const mjk_ctx* ctx, const mjk_basis* b, int n_j
         if (q.mode == 2) merged[q.base] = 1;
     for (const QClass& q : b->qcls)
         if (q.mode == 2 && q.nblocks > 0 &&
-            (q.nrt == 0 || q.kernel_pref == 2 || row_shmem(q, n_j, n_k, b->maxld) > ctx->row_lds))
+            (q.nrt == 0 || q.kernel_pref == 2 || !row_fits(ctx, q, n_j, n_k, b->maxld)))
             merged[q.base] = 0;
     std::vector<char> run(b->qcls.size(), 0);
     for (size_t c = 0; c < b->qcls.size(); ++c) {
@@ -1823,7 +1844,7 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
             a.tlist = nullptr;
             a.ntask = 0;
         }
-        const bool row_class = qc.mode == 2 || (row_shmem(qc, n_j, n_k, b->maxld) <= ctx->row_lds && qc.nrt > 0 &&
+        const bool row_class = qc.mode == 2 || (row_fits(ctx, qc, n_j, n_k, b->maxld) && qc.nrt > 0 &&
                                                 qc.kernel_pref != 2);
         if (compact && row_class && qc.nrt > 0 &&
             launch_compact_rows(qc.rt, qc.nrt, b->d_pmol, b->d_active, b->d_tlist + qc.tl0, b->d_tctl + ROW_TCTL * c, st))
@@ -1845,7 +1866,7 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
         // row kernel (one block per bra pair, K' rows in LDS) when the rows fit
         const size_t shmem = row_shmem(qc, n_j, n_k, b->maxld);
         int rr = -1;   // -1: no row kernel for this class (cooperative classes)
-        if (shmem <= ctx->row_lds && qc.nrt > 0 && qc.kernel_pref != 2) {
+        if (row_fits(ctx, qc, n_j, n_k, b->maxld) && qc.nrt > 0 && qc.kernel_pref != 2) {
             a.rt = qc.rt;
             a.rr = qc.rr;
             a.rtp = qc.rtp;
@@ -2172,7 +2193,7 @@ int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* b, int k, int n_dm, int hermi
     } else if (qc.mode == 2) {
         *kernel = 3;
     } else {
-        const bool row = row_shmem(qc, n_j, n_k, b->maxld) <= ctx->row_lds && qc.nrt > 0 && qc.kernel_pref != 2;
+        const bool row = row_fits(ctx, qc, n_j, n_k, b->maxld) && qc.nrt > 0 && qc.kernel_pref != 2;
         *kernel = row ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
     }
     return MJK_OK;

@@ -78,7 +78,7 @@ def test_merged_vs_split_and_mixed():
     b = opt.basis
     dm = densities(nao, 1, 33, True)
     vj, vk = b.get_jk(dm, 1, True, True, SCF_TOL)
-    lds = 98304
+    lds = 162816
     try:
         eng.check(eng.lib.mjk_ctx_set_row_lds(eng.handle, 0), 'mjk_ctx_set_row_lds')
         assert 3 not in [c[4] for c in b.classes() if c[3] > 0]

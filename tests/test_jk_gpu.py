@@ -172,7 +172,7 @@ def test_row_and_tile_kernels_agree(oracle, row_lds):
     try:
         got = [hf.get_jk(mol, dms[0], hermi=1, vhfopt=opt), hf.get_jk(mol, dms, hermi=0, vhfopt=opt)]
     finally:
-        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 98304) == 0
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 162816) == 0
     for (j0, k0), (j1, k1) in zip(ref, got):
         assert abs(j1 - j0).max() < 1e-11 * max(1, abs(j0).max())
         assert abs(k1 - k0).max() < 1e-11 * max(1, abs(k0).max())
@@ -279,12 +279,12 @@ def test_row_kernels_above_64k_lds(oracle):
     eng = _vhf.get_engine()
     out = {}
     try:
-        for lds in (147456, 0):
+        for lds in (162816, 0):
             assert eng.lib.mjk_ctx_set_row_lds(eng.handle, lds) == 0
             out[lds] = hf.get_jk(mol, dms, hermi=0, vhfopt=opt)
     finally:
-        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 98304) == 0
-    (j1, k1), (j0, k0) = out[147456], out[0]
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 162816) == 0
+    (j1, k1), (j0, k0) = out[162816], out[0]
     assert abs(j1 - j0).max() < 1e-11 * max(1, abs(j0).max())
     assert abs(k1 - k0).max() < 1e-11 * max(1, abs(k0).max())
     jo, ko = oracle.get_jk(mol, dms[1], hermi=0)
