@@ -62,11 +62,13 @@ def progress(msg):
 def kernel_name(cls, kind, rsh=False, ctr=(1, 1, 1, 1)):
     """Demangled name (as rocprofv3 prints it, less the argument list) of the
     kernel a quartet class ran on: 0 tile, 1 row, 2 cooperative, 3 merged row
-    kernel (contraction counts ctr); classes cut into slices run one kernel
-    per slice and are named by slice 0.  rsh: the row-kernel variant of the
-    range-separated operators."""
+    kernel (contraction counts ctr), 4 row kernel with the B rows of K' in
+    memory; classes cut into slices run one kernel per slice and are named by
+    slice 0.  rsh: the row-kernel variant of the range-separated operators."""
     from pyscf_amd.build import jk_split
     la, lb, lc, ld = cls
+    if kind == 4:
+        return 'mjk::k_jk_row_m<%d, %d, %d, %d, %d, 0>' % (la, lb, lc, ld, jk_split(la, lb, lc, ld))
     if kind == 3:
         return 'mjk::k_jk_row_gc<%d, %d, %d, %d, %d, %d, %d, %d>' % (tuple(cls) + tuple(ctr))
     if kind == 2:

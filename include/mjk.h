@@ -169,8 +169,9 @@ int mjk_get_jk_device(mjk_ctx* ctx, mjk_basis* basis, const double* d_dms, int n
  * ctr4 = contraction counts of the four shells (2 for a generally contracted
  * s shell kept whole, 1 otherwise), nblocks = launch size (0: no surviving
  * task), kernel = the kernel a build of n_dm densities with this hermi runs
- * for it (0 tile, 1 row, 2 cooperative, 3 merged row kernel; -1 the class
- * does not run: the other representation of its quartets does). */
+ * for it (0 tile, 1 row, 2 cooperative, 3 merged row kernel, 4 row kernel
+ * with only the A rows of K' in LDS; -1 the class does not run: the other
+ * representation of its quartets does). */
 int mjk_get_jk_resident(mjk_ctx* ctx, mjk_basis* basis, int n_dm, int hermi, int with_j, int with_k,
                         double direct_scf_tol, double fp32_tol, mjk_stats* stats);
 int mjk_basis_fetch_jk(mjk_ctx* ctx, mjk_basis* basis, int n_dm, double* vj, double* vk);
@@ -389,7 +390,9 @@ int mjk_profile_read(mjk_ctx* ctx, const mjk_basis* basis, int max_classes, int*
                      long long* nq, long long* nprimq, double* flops, long long* launches);
 /* Kernel each profiled class last ran on, in mjk_profile_read's order:
  * 0 tile kernel (k_jk), 1 row kernel (k_jk_row), 2 cooperative kernel
- * (k_jk_coop).  Returns the number of classes written. */
+ * (k_jk_coop), 3 merged row kernel (k_jk_row_gc), 4 row kernel with the B
+ * rows of K' in memory (k_jk_row_m).  Returns the number of classes
+ * written. */
 int mjk_profile_kinds(mjk_ctx* ctx, const mjk_basis* basis, int max_classes, int* kind);
 /* Accumulated GPU time of the quartet phase of the profiled builds (from the
  * end of the density transform to the end of the last class kernel, over all

@@ -79,6 +79,18 @@ def use_row(c):
     return jk_split(*c) <= 6 and (nf <= 120 or (nf <= 180 and nket <= 9))
 
 
+def use_row_m(c):
+    """Row classes that also get the kernel with only the A rows of K' in LDS
+    (the B rows' updates as memory atomics): f-shell bras with a second
+    shell above s, whose full rows pass a workgroup's 160 KB from about 1500
+    Cartesian functions (13 to 20 rows) while their 10 A rows fit to 2000,
+    cut into at most two slices.  Measured on MI355X (def2-TZVP chain, 1442
+    AOs, ms per launch, this kernel / the fallback): (fp|ps) 38.8 / 116.5
+    cooperative; (fp|ds) 70.6 / 50.7 and (fd|ps) 67.3 / 53.6 (five slices
+    each) stay on the fallback."""
+    return use_row(c) and c[0] == 3 and c[1] >= 1 and jk_split(*c) <= 2
+
+
 def use_coop(c):
     """The cooperative LDS kernel (one quartet per block) runs every class
     without a row kernel, and is the fallback of the row classes above 81
@@ -162,6 +174,16 @@ def generate():
                                  '{\n    return launch_jk_row<%d, %d, %d, %d, %d, %s>(nb, a, shmem, st);\n}\n'
                                  % ((tag, _name(c), sl) + c + (sl, rsh)))
                     files.append(_write('%s_%s%d.hip' % (_name(c), tag, u0), head + body + tail))
+    for c in classes:
+        if use_row_m(c):
+            split = jk_split(*c)
+            for u0 in range(0, split, SLICES_PER_UNIT):
+                body = ''
+                for sl in range(u0, min(split, u0 + SLICES_PER_UNIT)):
+                    body += ('int launch_rowm_%s_%d(int nb, const JKArgs& a, size_t shmem, hipStream_t st)\n'
+                             '{\n    return launch_jk_row_m<%d, %d, %d, %d, %d>(nb, a, shmem, st);\n}\n'
+                             % ((_name(c), sl) + c + (sl,)))
+                files.append(_write('%s_m%d.hip' % (_name(c), u0), head + body + tail))
     gck = gc_kernels()
     bybase = {}
     for k in gck:
@@ -201,11 +223,23 @@ def generate():
             for sl in range(split):
                 impl += '        rc |= launch_rowr_%s_%d(nb, a, shmem, st);\n' % (_name(c), sl)
             impl += '    }\n    return rc;\n}\n'
+        if use_row_m(c):
+            for sl in range(split):
+                decl += ('int launch_rowm_%s_%d(int nb, const JKArgs& a, size_t shmem, hipStream_t st);\n'
+                         % (_name(c), sl))
+            impl += ('static int launch_rowm_%s(int nb, const JKArgs& a, size_t shmem, hipStream_t st)\n{\n'
+                     '    int rc = 0;\n' % _name(c))
+            for sl in range(split):
+                impl += '    rc |= launch_rowm_%s_%d(nb, a, shmem, st);\n' % (_name(c), sl)
+            impl += '    return rc;\n}\n'
     key = lambda c: ((c[0] * 4 + c[1]) * 4 + c[2]) * 4 + c[3]
     rcases = ''.join('    case %d: return launch_row_%s(nblocks, a, shmem, st);\n' % (key(c), _name(c))
                      for c in classes if use_row(c))
     coop = ''.join('    case %d: return 1;\n' % key(c) for c in classes if use_coop(c))
     avail = ''.join('    case %d: return 1;\n' % key(c) for c in classes if use_row(c))
+    mcases = ''.join('    case %d: return launch_rowm_%s(nblocks, a, shmem, st);\n' % (key(c), _name(c))
+                     for c in classes if use_row_m(c))
+    mavail = ''.join('    case %d: return 1;\n' % key(c) for c in classes if use_row_m(c))
     cases = ''.join('    case %d: return launch_%s(nblocks, a, st);\n' % (key(c), _name(c)) for c in classes)
     gcases = ''.join('    case %d: return launch_rowg_%d(nblocks, a, shmem, st);\n' % (gc_key(k), gc_key(k))
                      for k in gck)
@@ -225,6 +259,13 @@ def generate():
             'int launch_jk_rows(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, size_t shmem, '
             'hipStream_t st)\n{\n'
             '    switch (((la * 4 + lb) * 4 + lc) * 4 + ld) {\n' + rcases +
+            '    default: return -1;\n    }\n}\n'
+            'int row_m_available(int la, int lb, int lc, int ld)\n{\n'
+            '    switch (((la * 4 + lb) * 4 + lc) * 4 + ld) {\n' + mavail +
+            '    default: return 0;\n    }\n}\n'
+            'int launch_jk_rows_m(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, size_t shmem, '
+            'hipStream_t st)\n{\n'
+            '    switch (((la * 4 + lb) * 4 + lc) * 4 + ld) {\n' + mcases +
             '    default: return -1;\n    }\n}\n'
             'int row_gc_available(int la, int lb, int lc, int ld, int ca, int cb, int cc, int cd)\n{\n'
             '    if (ca < 1 || ca > 2 || cb < 1 || cb > 2 || cc < 1 || cc > 2 || cd < 1 || cd > 2) return 0;\n'

@@ -947,7 +947,17 @@ __device__ __forceinline__ void lds_add(double* p, double v)
     atomicAdd(p, v);
 }
 
-template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R, bool RSH>
+// K' updates of the bra's B rows: in LDS, or (KBM, the rows of large
+// molecules that do not fit beside the A rows) memory-side atomics into the
+// accumulator copy.
+template <bool KBM>
+__device__ __forceinline__ void kb_add(double* p, double v)
+{
+    if constexpr (KBM) atomic_add(p, v);
+    else lds_add(p, v);
+}
+
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, typename R, bool RSH, bool KBM = false>
 __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c, double* kr, double* jr,
                                            int krow, long long base, int ld, int oa, int ob, int cp)
 {
@@ -1006,7 +1016,8 @@ __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c,
     for (int s = 0; s < a.n_k; ++s) {
         const double* __restrict__ D = a.Dk + s * a.slot + base;
         double* KA = kr + s * krow;              // rows a: K'[a, :]
-        double* KB = KA + NA * ld;               // rows b: K'[b, :]
+        double* KB = KBM ? a.K + cp * a.kcopy + s * a.slot + base + (long long)ob * ld   // rows b: K'[b, :]
+                         : KA + NA * ld;
         // K'[a,d] += sum_bc v D[b,c];  K'[b,d] += sum_ac v D[a,c]
         // K'[a,c] += sum_bd v D[b,d];  K'[b,c] += sum_ad v D[a,d]
         double dbc[MB][MC], dac[MA][MC], dbd[MB][ND], dad[MA][ND];
@@ -1046,7 +1057,7 @@ __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c,
                 for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
                     for (int ic = IC0; ic < IC1; ++ic) acc = fma(GOUT(ia, ib2, ic, id), dac[ia - IA0][ic - IC0], acc);
-                lds_add(KB + ib2 * ld + od + id, acc);
+                kb_add<KBM>(KB + ib2 * ld + od + id, acc);
             }
 #pragma unroll
         for (int ia = IA0; ia < IA1; ++ia)
@@ -1068,7 +1079,7 @@ __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c,
                 for (int ia = IA0; ia < IA1; ++ia)
 #pragma unroll
                     for (int id = 0; id < ND; ++id) acc = fma(GOUT(ia, ib2, ic, id), dad[ia - IA0][id], acc);
-                lds_add(KB + ib2 * ld + oc + ic, acc);
+                kb_add<KBM>(KB + ib2 * ld + oc + ic, acc);
             }
     }
 }
@@ -1086,14 +1097,15 @@ struct RowOccupancy {
         (LA == 2 && LB == 1 && LC == 1 && LD == 0) || (LA == 2 && LB == 0 && LC == 1 && LD == 1) ? 2 : MJK_ROW_WPE;
 };
 
-// RSH: the variant for the range-separated operators (a.omega != 0); the
-// Coulomb variant keeps its registers free of that path.
-template <int LA, int LB, int LC, int LD, int SPLIT, int S, bool RSH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSH ? 1 : RowOccupancy<LA, LB, LC, LD>::value)))
-void k_jk_row(const JKArgs a)
+// Body of the row kernels.  RSH: the variant for the range-separated
+// operators (a.omega != 0); the Coulomb variant keeps its registers free of
+// that path.  KBM: only the A rows of K' in LDS, the B rows' updates as
+// memory-side atomics (kb_add).
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, bool RSH, bool KBM>
+__device__ __forceinline__ void jk_row_body(const JKArgs& a)
 {
     constexpr int NA = ncart(LA), NB = ncart(LB);
-    constexpr int NROW = NA + NB;
+    constexpr int NROW = KBM ? NA : NA + NB;   // K' rows held in LDS
     extern __shared__ double lds_rows[];
     __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
     __shared__ int s_next;   // next unclaimed ket of the chunk (waves take 64 at a time)
@@ -1213,8 +1225,8 @@ void k_jk_row(const JKArgs a)
             cnt[use32 ? 2 : 0] += quartet_tally;
             cnt[use32 ? 3 : 1] += (unsigned long long)(c.npb * c.npk);
         }
-        if (use32) row_digest<LA, LB, LC, LD, SPLIT, S, float, RSH>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
-        else row_digest<LA, LB, LC, LD, SPLIT, S, double, RSH>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
+        if (use32) row_digest<LA, LB, LC, LD, SPLIT, S, float, RSH, KBM>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
+        else row_digest<LA, LB, LC, LD, SPLIT, S, double, RSH, KBM>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
     }
     __syncthreads();
     // flush the accumulated rows of K' and the J'[a,b] block
@@ -1252,11 +1264,34 @@ void k_jk_row(const JKArgs a)
     }
 }
 
+template <int LA, int LB, int LC, int LD, int SPLIT, int S, bool RSH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSH ? 1 : RowOccupancy<LA, LB, LC, LD>::value)))
+void k_jk_row(const JKArgs a)
+{
+    jk_row_body<LA, LB, LC, LD, SPLIT, S, RSH, false>(a);
+}
+
+// Row kernel of classes whose full K' rows exceed a workgroup's LDS in large
+// molecules (f-shell bras): the A rows in LDS, the B rows in memory.
+template <int LA, int LB, int LC, int LD, int SPLIT, int S>
+__global__ __launch_bounds__(256) void k_jk_row_m(const JKArgs a)
+{
+    jk_row_body<LA, LB, LC, LD, SPLIT, S, false, true>(a);
+}
+
 template <int LA, int LB, int LC, int LD, int S, bool RSH>
 int launch_jk_row(int nblocks, const JKArgs& a, size_t shmem, hipStream_t st)
 {
     constexpr int SPLIT = jk_split(LA, LB, LC, LD);
     hipLaunchKernelGGL((k_jk_row<LA, LB, LC, LD, SPLIT, S, RSH>), dim3(nblocks), dim3(256), shmem, st, a);
+    return (int)hipGetLastError();
+}
+
+template <int LA, int LB, int LC, int LD, int S>
+int launch_jk_row_m(int nblocks, const JKArgs& a, size_t shmem, hipStream_t st)
+{
+    constexpr int SPLIT = jk_split(LA, LB, LC, LD);
+    hipLaunchKernelGGL((k_jk_row_m<LA, LB, LC, LD, SPLIT, S>), dim3(nblocks), dim3(256), shmem, st, a);
     return (int)hipGetLastError();
 }
 

@@ -48,6 +48,8 @@ int row_kernel_available(int la, int lb, int lc, int ld);
 int launch_jk_rows_gc(int la, int lb, int lc, int ld, int ca, int cb, int cc, int cd, int nblocks, const JKArgs& a,
                       size_t shmem, hipStream_t st);
 int row_gc_available(int la, int lb, int lc, int ld, int ca, int cb, int cc, int cd);
+int row_m_available(int la, int lb, int lc, int ld);
+int launch_jk_rows_m(int la, int lb, int lc, int ld, int nblocks, const JKArgs& a, size_t shmem, hipStream_t st);
 int coop_kernel(int la, int lb, int lc, int ld);
 int launch_qcond(int la, int lb, const QCondArgs& a, hipStream_t st);
 int upload_c2s(hipStream_t st);
@@ -457,7 +459,8 @@ struct mjk_basis {
     std::vector<hipEvent_t> pev;          // 2 per quartet class
     std::vector<double> prof_ms;
     std::vector<long long> prof_nq, prof_np, prof_launches;
-    std::vector<int> prof_kind;           // kernel of the class's last launch: 0 tile, 1 row, 2 cooperative
+    std::vector<int> prof_kind;           // kernel of the class's last launch: 0 tile, 1 row, 2 cooperative,
+                                          // 3 merged row, 4 row with the B rows in memory
     double prof_span_ms = 0;              // quartet phase of the profiled builds (all classes, all streams)
     long long prof_builds = 0;
 };
@@ -859,6 +862,26 @@ bool row_fits(const mjk_ctx* ctx, const QClass& q, int n_j, int n_k, int maxld)
 {
     const size_t sh = row_shmem(q, n_j, n_k, maxld);
     return sh <= ctx->row_lds && sh + row_static_lds(q) <= (size_t)LDS_WORKGROUP;
+}
+
+// Dynamic LDS of the row kernel that keeps only the A rows of K' (the B rows'
+// updates as memory-side atomics): plain classes of f-shell bras.
+size_t row_shmem_m(const QClass& q, int n_j, int n_k, int maxld)
+{
+    const size_t ra = ncart(q.la), rb = ncart(q.lb);
+    return sizeof(double) * ((size_t)n_k * ra * maxld + (size_t)n_j * ra * rb);
+}
+
+// Row kernel a plain class runs in a build: 1 with its full K' rows in LDS,
+// 2 with only the A rows there (the Coulomb operator; when the full rows do
+// not fit), 0 none (its fallback kernel).
+int row_choice(const mjk_ctx* ctx, const mjk_basis* b, const QClass& q, int n_j, int n_k)
+{
+    if (q.nrt == 0 || q.kernel_pref == 2) return 0;
+    if (row_fits(ctx, q, n_j, n_k, b->maxld)) return 1;
+    if (q.mode != 0 || b->omega != 0.0 || !row_m_available(q.la, q.lb, q.lc, q.ld)) return 0;
+    const size_t sh = row_shmem_m(q, n_j, n_k, b->maxld);
+    return sh <= ctx->row_lds && sh + row_static_lds(q) <= (size_t)LDS_WORKGROUP ? 2 : 0;
 }
 
 int parse_molecule(mjk_ctx* ctx, const int* atm, int natm, const int* bas, int nbas, const double* env,
@@ -1844,8 +1867,8 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
             a.tlist = nullptr;
             a.ntask = 0;
         }
-        const bool row_class = qc.mode == 2 || (row_fits(ctx, qc, n_j, n_k, b->maxld) && qc.nrt > 0 &&
-                                                qc.kernel_pref != 2);
+        const int rowk = qc.mode == 2 ? 0 : row_choice(ctx, b, qc, n_j, n_k);
+        const bool row_class = qc.mode == 2 || rowk > 0;
         if (compact && row_class && qc.nrt > 0 &&
             launch_compact_rows(qc.rt, qc.nrt, b->d_pmol, b->d_active, b->d_tlist + qc.tl0, b->d_tctl + ROW_TCTL * c, st))
             return fail(ctx, MJK_EDEVICE, "task compaction launch failed");
@@ -1864,19 +1887,20 @@ static int get_jk_impl(mjk_ctx* ctx, mjk_basis* b, const double* d_dms, int n_dm
             continue;
         }
         // row kernel (one block per bra pair, K' rows in LDS) when the rows fit
-        const size_t shmem = row_shmem(qc, n_j, n_k, b->maxld);
         int rr = -1;   // -1: no row kernel for this class (cooperative classes)
-        if (row_fits(ctx, qc, n_j, n_k, b->maxld) && qc.nrt > 0 && qc.kernel_pref != 2) {
+        if (rowk > 0) {
             a.rt = qc.rt;
             a.rr = qc.rr;
             a.rtp = qc.rtp;
             a.rprim = qc.rprim;
-            rr = launch_jk_rows(qc.la, qc.lb, qc.lc, qc.ld, nrow_blocks, a, shmem, st);
+            rr = rowk == 1 ? launch_jk_rows(qc.la, qc.lb, qc.lc, qc.ld, nrow_blocks, a, row_shmem(qc, n_j, n_k, b->maxld), st)
+                           : launch_jk_rows_m(qc.la, qc.lb, qc.lc, qc.ld, nrow_blocks, a,
+                                              row_shmem_m(qc, n_j, n_k, b->maxld), st);
             if (rr > 0) return fail(ctx, MJK_EDEVICE, "row kernel launch failed for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
         }
         if (rr != 0 && launch_jk(qc.la, qc.lb, qc.lc, qc.ld, (int)qc.nblocks, a, st))
             return fail(ctx, MJK_EUNSUPPORTED, "no kernel for class (%d%d|%d%d)", qc.la, qc.lb, qc.lc, qc.ld);
-        if (prof) b->prof_kind[c] = rr == 0 ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
+        if (prof) b->prof_kind[c] = rr == 0 ? (rowk == 2 ? 4 : 1) : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
         if (prof) HIPCHK(ctx, hipEventRecord(b->pev[2 * c + 1], st));
     }
     for (int i = 1; i < nst; ++i) {
@@ -2193,8 +2217,8 @@ int mjk_basis_class(mjk_ctx* ctx, const mjk_basis* b, int k, int n_dm, int hermi
     } else if (qc.mode == 2) {
         *kernel = 3;
     } else {
-        const bool row = row_fits(ctx, qc, n_j, n_k, b->maxld) && qc.nrt > 0 && qc.kernel_pref != 2;
-        *kernel = row ? 1 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
+        const int rowk = row_choice(ctx, b, qc, n_j, n_k);
+        *kernel = rowk == 1 ? 1 : rowk == 2 ? 4 : coop_kernel(qc.la, qc.lb, qc.lc, qc.ld) ? 2 : 0;
     }
     return MJK_OK;
 }

@@ -200,7 +200,8 @@ class Basis:
         """[(index, (la, lb, lc, ld), (ca, cb, cc, cd), nblocks, kernel)] of the
         class list; ctr = contraction counts of the shells (2: a generally
         contracted s shell kept whole); kernel 0 tile, 1 row, 2 cooperative,
-        3 merged row kernel, -1 the class does not run in such a build."""
+        3 merged row kernel, 4 row kernel with the B rows of K' in memory, -1
+        the class does not run in such a build."""
         L = self.engine.lib
         out = []
         l4 = numpy.zeros(4, dtype=numpy.int32)

@@ -143,6 +143,35 @@ def test_fshells_def2_tzvp(oracle):
     assert abs(vj - j0).max() < TOL * 10 and abs(vk - k0).max() < TOL * 10
 
 
+def test_row_kernel_b_rows_in_memory(oracle):
+    """With the row-kernel LDS limit just above the 10 A rows of an f-shell bra
+    (and below its full 13-20 rows), the (fp|..), (fd|..) and (ff|ss) classes
+    run the row kernel that keeps the B rows of K' in memory; J and K equal
+    the oracle's, for one density (one K' set) and for a non-symmetric one
+    (two)."""
+    mol = gto.M(atom='O 0 0 0; H 0 -0.757 0.587; H 0 0.757 0.587; O 2.5 0.3 0.4', basis='def2-tzvp')
+    nao = mol.nao_nr()
+    bas = numpy.asarray(mol._bas)
+    ncart = int(sum((l + 1) * (l + 2) // 2 * nc for l, nc in zip(bas[:, 1], bas[:, 3])))
+    rng = numpy.random.default_rng(13)
+    c = rng.standard_normal((nao, 8))
+    eng = _vhf.get_engine()
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    try:
+        for hermi, n_k, dm in ((1, 1, c @ c.T / 8), (0, 2, rng.standard_normal((nao, nao)))):
+            lds = 8 * (n_k * 10 * ncart + 10 * 10)
+            assert eng.lib.mjk_ctx_set_row_lds(eng.handle, lds) == 0
+            kinds = {cl[1]: cl[4] for cl in opt.basis.classes(1, hermi)}
+            assert kinds[(3, 1, 0, 0)] == 4 and kinds[(3, 1, 1, 0)] == 4 and kinds[(3, 3, 0, 0)] == 4, kinds
+            vj, vk = hf.get_jk(mol, dm, hermi=hermi, vhfopt=opt)
+            j0, k0 = oracle.get_jk(mol, dm, hermi=hermi)
+            print('B rows in memory, hermi=%d: max|dJ|=%.2e max|dK|=%.2e' % (hermi, abs(vj - j0).max(),
+                                                                         abs(vk - k0).max()), flush=True)
+            assert abs(vj - j0).max() < TOL * max(1, abs(j0).max()) and abs(vk - k0).max() < TOL * max(1, abs(k0).max())
+    finally:
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 162816) == 0
+
+
 def test_cooperative_classes_dd(oracle):
     # (dd|dd) and (dd|dp) run the cooperative LDS kernel
     mol = gto.M(atom='O 0 0 0; O 0 0 1.2; O 1.1 0.2 0.3', basis='def2-svp')
