@@ -1097,186 +1097,24 @@ struct RowOccupancy {
         (LA == 2 && LB == 1 && LC == 1 && LD == 0) || (LA == 2 && LB == 0 && LC == 1 && LD == 1) ? 2 : MJK_ROW_WPE;
 };
 
-// Body of the row kernels.  RSH: the variant for the range-separated
-// operators (a.omega != 0); the Coulomb variant keeps its registers free of
-// that path.  KBM: only the A rows of K' in LDS, the B rows' updates as
-// memory-side atomics (kb_add).
-template <int LA, int LB, int LC, int LD, int SPLIT, int S, bool RSH, bool KBM>
-__device__ __forceinline__ void jk_row_body(const JKArgs& a)
-{
-    constexpr int NA = ncart(LA), NB = ncart(LB);
-    constexpr int NROW = KBM ? NA : NA + NB;   // K' rows held in LDS
-    extern __shared__ double lds_rows[];
-    __shared__ int s_start[ROW_MAXBUCKET], s_pre[ROW_MAXBUCKET + 1];
-    __shared__ int s_next;   // next unclaimed ket of the chunk (waves take 64 at a time)
-    const int tid = threadIdx.x, lane = tid & 63;
-    using RT = RowRys<LA, LB, LC, LD>;
-    __shared__ double s_rys[RT::size > 0 ? RT::size : 1];
-    if constexpr (RT::size > 0) {
-        rys_lds_fill<RT::nr>(s_rys, a.rys, tid, 256);
-        __syncthreads();
-    }
-    __shared__ int s_task;
-    unsigned long long cnt[4] = {0, 0, 0, 0};
-    const bool persist = a.thead != nullptr;
-    const int ntask = persist ? (a.tcount ? *a.tcount : a.ntask) : 0;
-    for (int it = 0;; ++it) {
-    int tb;
-    if (persist) {
-        if (tid == 0) s_task = atomicAdd(a.thead + S, 1);   // each slice kernel has its own counter
-        __syncthreads();
-        const int ti = __builtin_amdgcn_readfirstlane(s_task);
-        __syncthreads();
-        if (ti >= ntask) break;   // block-uniform
-        tb = a.tlist ? uload(a.tlist + ti) : ti;
-    } else {
-        if (it > 0) break;
-        tb = blockIdx.x;
-    }
-    const int4 task = uload(a.rt + tb);
-    const int p = task.x;
-    const int mol = uload(a.pmol + p);
-    if (a.mol_active && !uload(a.mol_active + mol)) continue;   // block-uniform
-    const int ld = uload(a.M.ld + mol);
-    const long long base = uload(a.M.dbase + mol);
-    const int krow = NROW * ld;
-    double* kr = lds_rows;
-    double* jr = lds_rows + a.n_k * krow;
-    const int nz = a.n_k * krow + a.n_j * NA * NB;
-    for (int e = tid; e < nz; e += 256) lds_rows[e] = 0.0;
-    const double qp = uload(a.P.rA + p).w;
-    // surviving ket range of every primitive bucket (Schwarz prefix of the
-    // descending segment, q <= p on the diagonal class), from the task list
-    if (tid < a.nbucket) {
-        const int2 r = a.rr[(long long)task.w * a.nbucket + tid];
-        s_start[tid] = r.x;
-        s_pre[tid + 1] = r.y - r.x;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        s_pre[0] = 0;
-        for (int k = 1; k <= a.nbucket; ++k) s_pre[k] += s_pre[k - 1];
-        s_next = task.y;
-    }
-    __syncthreads();
-    const int iend = min(s_pre[a.nbucket], task.z);
-
-    QuartetCtx c;
-    c.p = p;
-    c.mol = mol;
-    c.pvalid = true;
-    c.rtab = s_rys;
-    c.rA = uload(a.P.rA + p);
-    c.AB = uload(a.P.rAB + p);
-    const int pbo = uload(a.rtp + tb);   // this task's share of the bra's primitive pairs
-    c.pb0 = uload(a.P.prim0 + p) + pbo;
-    c.npb = min(a.rprim, uload(a.P.nprim + p) - pbo);
-    const unsigned long long quartet_tally = pbo == 0;   // a quartet counts once, in its first share
-    const bool diagA = uload(a.P.shA + p) == uload(a.P.shB + p);
-    const int oa = uload(a.P.cartA + p), ob = uload(a.P.cartB + p);
-    const int cp = xcc_copy(a.ncopy);
-    for (;;) {
-        int i0 = 0;
-        if (lane == 0) i0 = atomicAdd(&s_next, 64);
-        i0 = __shfl(i0, 0);
-        if (i0 >= iend) break;   // wave-uniform
-        const int idx = i0 + lane;
-        c.qvalid = idx < iend;
-        c.q = 0;
-        if (c.qvalid) {
-            int lo = 0, hi = a.nbucket;   // bucket k with s_pre[k] <= idx < s_pre[k + 1]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_pre[mid] <= idx) lo = mid; else hi = mid;
-            }
-            c.q = s_start[lo] + idx - s_pre[lo];
-        }
-        c.alive = c.qvalid;
-        c.rC = c.CD = make_double4(0, 0, 0, 0);
-        c.pk0 = c.npk = 0;
-        c.tq = c.ts = 0;
-        bool small = true;
-        if (c.alive) {
-            c.rC = a.P.rA[c.q];
-            const double qpq = qp * c.rC.w;
-            if (a.screen || a.fp32_tol > 0) {
-                const double dmax = quartet_dmax(a, mol, p, c.q);
-                if (a.screen) c.alive = qpq > a.cutoff && dmax * qpq > a.cutoff;
-                small = dmax * qpq < a.fp32_tol;
-            } else {
-                small = false;
-            }
-        }
-        c.fac = 0.0;
-        if (c.alive) {
-            c.pk0 = a.P.prim0[c.q];
-            c.npk = a.P.nprim[c.q];
-            c.tq = a.P.tpos[c.q];
-            c.ts = a.P.tstride[c.q];
-            c.CD = a.P.rAB[c.q];
-            double f = diagA ? 0.5 : 1.0;
-            if (a.P.shA[c.q] == a.P.shB[c.q]) f *= 0.5;
-            if (c.q == p && a.diag) f *= 0.5;
-            c.fac = f;
-        }
-        if (!__ballot(c.alive)) continue;   // wave-uniform
-        const bool use32 = a.fp32_tol > 0 && __ballot(c.alive && !small) == 0;
-        if (c.alive) {
-            cnt[use32 ? 2 : 0] += quartet_tally;
-            cnt[use32 ? 3 : 1] += (unsigned long long)(c.npb * c.npk);
-        }
-        if (use32) row_digest<LA, LB, LC, LD, SPLIT, S, float, RSH, KBM>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
-        else row_digest<LA, LB, LC, LD, SPLIT, S, double, RSH, KBM>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
-    }
-    __syncthreads();
-    // flush the accumulated rows of K' and the J'[a,b] block
-    for (int e = tid; e < a.n_k * krow; e += 256) {
-        const double v = kr[e];
-        if (v == 0.0) continue;
-        const int s = e / krow;
-        const int r = (e - s * krow) / ld;
-        const int col = e - s * krow - r * ld;
-        const int row = r < NA ? oa + r : ob + (r - NA);
-        atomic_add(a.K + cp * a.kcopy + s * a.slot + base + (long long)row * ld + col, v);
-    }
-    for (int e = tid; e < a.n_j * NA * NB; e += 256) {
-        const double v = jr[e];
-        if (v == 0.0) continue;
-        const int s = e / (NA * NB);
-        const int ia = (e / NB) % NA, ib2 = e % NB;
-        atomic_add(a.J + cp * a.jcopy + s * a.slot + base + (long long)(oa + ia) * ld + ob + ib2, v);
-    }
-    __syncthreads();   // the next task rewrites the LDS rows
-    }
-    if (S == 0 && a.counters) {   // one atomic per counter and block
-        __shared__ unsigned long long s_cnt[4];
-        if (tid < 4) s_cnt[tid] = 0;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            unsigned long long v = cnt[k];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-            if (lane == 0 && v) atomicAdd(s_cnt + k, v);
-        }
-        __syncthreads();
-        if (tid < 4 && s_cnt[tid]) atomicAdd(a.counters + tid, s_cnt[tid]);
-    }
-}
-
+// Row kernels.  RSH: the variant for the range-separated operators
+// (a.omega != 0); the Coulomb variant keeps its registers free of that path.
 template <int LA, int LB, int LC, int LD, int SPLIT, int S, bool RSH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RSH ? 1 : RowOccupancy<LA, LB, LC, LD>::value)))
 void k_jk_row(const JKArgs a)
 {
-    jk_row_body<LA, LB, LC, LD, SPLIT, S, RSH, false>(a);
+    constexpr bool KBM = false;
+#include "jk_row_body.inc"
 }
 
 // Row kernel of classes whose full K' rows exceed a workgroup's LDS in large
-// molecules (f-shell bras): the A rows in LDS, the B rows in memory.
+// molecules (f-shell bras): the A rows in LDS, the B rows' updates as
+// memory-side atomics.
 template <int LA, int LB, int LC, int LD, int SPLIT, int S>
 __global__ __launch_bounds__(256) void k_jk_row_m(const JKArgs a)
 {
-    jk_row_body<LA, LB, LC, LD, SPLIT, S, false, true>(a);
+    constexpr bool KBM = true, RSH = false;
+#include "jk_row_body.inc"
 }
 
 template <int LA, int LB, int LC, int LD, int S, bool RSH>
