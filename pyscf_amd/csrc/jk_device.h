@@ -1420,7 +1420,7 @@ __global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
         __syncthreads();
     }
     __shared__ int s_task;
-    unsigned long long cnt[4] = {0, 0, 0, 0};
+    unsigned int cnt[4] = {0, 0, 0, 0};   // per-lane work tallies: a lane's share of a launch stays far below 2^32
     const bool persist = a.thead != nullptr;
     const int ntask = persist ? (a.tcount ? *a.tcount : a.ntask) : 0;
     for (int it = 0;; ++it) {
@@ -1524,8 +1524,8 @@ __global__ __launch_bounds__(256) void k_jk_row_gc(const JKArgs a)
         if (!__ballot(c.alive)) continue;   // wave-uniform
         const bool use32 = a.fp32_tol > 0 && __ballot(c.alive && !small) == 0;
         if (c.alive) {
-            cnt[use32 ? 2 : 0] += quartet_tally;
-            cnt[use32 ? 3 : 1] += (unsigned long long)(c.npb * c.npk);
+            cnt[use32 ? 2 : 0] += (unsigned)quartet_tally;
+            cnt[use32 ? 3 : 1] += (unsigned)(c.npb * c.npk);
         }
         if (use32) row_digest_gc<LA, LB, LC, LD, CA, CB, CC, CD, float>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
         else row_digest_gc<LA, LB, LC, LD, CA, CB, CC, CD, double>(a, c, kr, jr, krow, base, ld, oa, ob, cp);
