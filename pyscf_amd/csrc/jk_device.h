@@ -239,6 +239,29 @@ __device__ __forceinline__ double tile_sum(double v)
     return v;
 }
 
+// Sum of v over the 64 lanes of a full wave, left in lane 63, by DPP moves on
+// the VALU (no LDS traffic): quad swaps, half-row and row mirrors, then the
+// row broadcasts of lanes 15 and 31 into the odd rows and the upper half.
+// Lanes that a step does not write add 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double wave_sum_last(double v)
+{
+    v += dpp_move<0xB1, 0xf>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4E, 0xf>(v);    // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x141, 0xf>(v);   // row_half_mirror
+    v += dpp_move<0x140, 0xf>(v);   // row_mirror
+    v += dpp_move<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_move<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // Component ranges of slice S of a class cut into SPLIT slices.
 template <int LA, int LB, int LC, int LD, int SPLIT, int S>
 struct SliceRange {
@@ -1007,9 +1030,8 @@ __device__ __forceinline__ void row_digest(const JKArgs& a, const QuartetCtx& c,
                 for (int ic = IC0; ic < IC1; ++ic)
 #pragma unroll
                     for (int id = 0; id < ND; ++id) acc = fma(GOUT(ia, ib2, ic, id), dcd[ic - IC0][id], acc);
-                acc = tile_sum<1>(acc);
-                acc = tile_sum<8>(acc);
-                if (lane == 0 && acc != 0.0) lds_add(jr + s * NA * NB + ia * NB + ib2, acc);
+                acc = wave_sum_last(acc);
+                if (lane == 63 && acc != 0.0) lds_add(jr + s * NA * NB + ia * NB + ib2, acc);
             }
     }
     if (!c.alive) return;
@@ -1289,9 +1311,8 @@ __device__ __forceinline__ void row_digest_gc(const JKArgs& a, const QuartetCtx&
 #pragma unroll
             for (int id = 0; id < ND; ++id)
                 acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dcd[ck * NC + ic][cl * ND + id], acc);
-            acc = tile_sum<1>(acc);
-            acc = tile_sum<8>(acc);
-            if (lane == 0 && acc != 0.0) lds_add(jr + s * NRA * NRB + (ci * NA + ia) * NRB + cj * NB + ib2, acc);
+            acc = wave_sum_last(acc);
+            if (lane == 63 && acc != 0.0) lds_add(jr + s * NRA * NRB + (ci * NA + ia) * NRB + cj * NB + ib2, acc);
         }
     }
     if (!c.alive) return;
