@@ -1243,6 +1243,19 @@ __device__ __forceinline__ void eri_gc(const JKArgs& a, const QuartetCtx& c,
         for (int f = 0; f < NF; ++f) gd[k][f] = double(gout[k][f]) * c.fac;
 }
 
+// Merged patterns whose K digestion loads its four D blocks one by one: the
+// registers of the blocks not yet needed then stay free, and the kernel
+// drops below 168 VGPRs (three waves per SIMD).  Measured on MI355X
+// (C30H62 cc-pVDZ, each class alone): (ps|ps) with a merged bra s 1.14x and
+// (pp|ss) with a merged ket s 1.20x faster; (ds|ss) and (pp|ps) patterns
+// lost 3-19% to the serialised loads and keep them together.
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD>
+struct GcStagedLoads {
+    static constexpr bool value =
+        (LA == 1 && LB == 0 && LC == 1 && LD == 0 && CA == 1 && CB == 2 && CC == 1 && CD == 1) ||
+        (LA == 1 && LB == 1 && LC == 0 && LD == 0 && CA == 1 && CB == 1 && CC == 2 && CD == 1);
+};
+
 template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, typename R>
 __device__ __forceinline__ void row_digest_gc(const JKArgs& a, const QuartetCtx& c, double* kr, double* jr,
                                               int krow, long long base, int ld, int oa, int ob, int cp)
@@ -1320,23 +1333,28 @@ __device__ __forceinline__ void row_digest_gc(const JKArgs& a, const QuartetCtx&
         const double* __restrict__ D = a.Dk + s * a.slot + base;
         double* KA = kr + s * krow;              // rows a (all contractions of A): K'[a, :]
         double* KB = KA + NRA * ld;              // rows b: K'[b, :]
+        // the four D blocks: loaded together, or (GcStagedLoads) each just before
+        // its products, the scheduling barriers keeping the loads apart
+        constexpr bool STAGED = GcStagedLoads<LA, LB, LC, LD, CA, CB, CC, CD>::value;
         double dbc[NRB][CC * NC], dac[NRA][CC * NC], dbd[NRB][CD * ND], dad[NRA][CD * ND];
 #pragma unroll
         for (int i = 0; i < NRB; ++i)
 #pragma unroll
             for (int j = 0; j < CC * NC; ++j) dbc[i][j] = D[(ob + i) * ld + oc + j];
+        if constexpr (!STAGED) {
 #pragma unroll
-        for (int i = 0; i < NRA; ++i)
+            for (int i = 0; i < NRA; ++i)
 #pragma unroll
-            for (int j = 0; j < CC * NC; ++j) dac[i][j] = D[(oa + i) * ld + oc + j];
+                for (int j = 0; j < CC * NC; ++j) dac[i][j] = D[(oa + i) * ld + oc + j];
 #pragma unroll
-        for (int i = 0; i < NRB; ++i)
+            for (int i = 0; i < NRB; ++i)
 #pragma unroll
-            for (int j = 0; j < CD * ND; ++j) dbd[i][j] = D[(ob + i) * ld + od + j];
+                for (int j = 0; j < CD * ND; ++j) dbd[i][j] = D[(ob + i) * ld + od + j];
 #pragma unroll
-        for (int i = 0; i < NRA; ++i)
+            for (int i = 0; i < NRA; ++i)
 #pragma unroll
-            for (int j = 0; j < CD * ND; ++j) dad[i][j] = D[(oa + i) * ld + od + j];
+                for (int j = 0; j < CD * ND; ++j) dad[i][j] = D[(oa + i) * ld + od + j];
+        }
         // K'[a,d] += sum_bc v D[b,c]
 #pragma unroll
         for (int ci = 0; ci < CA; ++ci)
@@ -1357,6 +1375,13 @@ __device__ __forceinline__ void row_digest_gc(const JKArgs& a, const QuartetCtx&
             for (int ic = 0; ic < NC; ++ic)
                 acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dbc[cj * NB + ib2][ck * NC + ic], acc);
             lds_add(KA + (ci * NA + ia) * ld + od + cl * ND + id, acc);
+        }
+        if constexpr (STAGED) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < NRA; ++i)
+#pragma unroll
+                for (int j = 0; j < CC * NC; ++j) dac[i][j] = D[(oa + i) * ld + oc + j];
         }
         // K'[b,d] += sum_ac v D[a,c]
 #pragma unroll
@@ -1379,6 +1404,13 @@ __device__ __forceinline__ void row_digest_gc(const JKArgs& a, const QuartetCtx&
                 acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dac[ci * NA + ia][ck * NC + ic], acc);
             lds_add(KB + (cj * NB + ib2) * ld + od + cl * ND + id, acc);
         }
+        if constexpr (STAGED) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < NRB; ++i)
+#pragma unroll
+                for (int j = 0; j < CD * ND; ++j) dbd[i][j] = D[(ob + i) * ld + od + j];
+        }
         // K'[a,c] += sum_bd v D[b,d]
 #pragma unroll
         for (int ci = 0; ci < CA; ++ci)
@@ -1399,6 +1431,13 @@ __device__ __forceinline__ void row_digest_gc(const JKArgs& a, const QuartetCtx&
             for (int id = 0; id < ND; ++id)
                 acc = fma(GV(ci, cj, ck, cl, ia, ib2, ic, id), dbd[cj * NB + ib2][cl * ND + id], acc);
             lds_add(KA + (ci * NA + ia) * ld + oc + ck * NC + ic, acc);
+        }
+        if constexpr (STAGED) {
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < NRA; ++i)
+#pragma unroll
+                for (int j = 0; j < CD * ND; ++j) dad[i][j] = D[(oa + i) * ld + od + j];
         }
         // K'[b,c] += sum_ad v D[a,d]
 #pragma unroll
