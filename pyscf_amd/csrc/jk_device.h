@@ -648,6 +648,21 @@ __global__ __launch_bounds__(64) void k_jk(const JKArgs a)
 // Digestion: the contracted block goes to LDS (reusing the I region), the
 // D blocks are staged, and each J'/K' element is computed by one owner
 // thread and added with one FP64 atomic.
+// Cooperative classes run at three waves per SIMD (measured on MI355X, ms per
+// launch at two -> three waves): (dp|dp) C30H62 cc-pVDZ 2.37 -> 2.16, (dd|dp)
+// 1.11 -> 0.87, (dd|pp) 1.45 -> 1.39; (fp|dp) and (fd|pp) of a def2-TZVP
+// chain 1.23x and 1.22x.  Classes already at three waves by registers lost up
+// to 7% to the smaller primitive batches that three waves need in LDS.
+__host__ __device__ constexpr int coop_key(int la, int lb, int lc, int ld) { return ((la * 4 + lb) * 4 + lc) * 4 + ld; }
+__host__ __device__ constexpr bool coop_three_waves(int la, int lb, int lc, int ld)
+{
+    const int k = coop_key(la, lb, lc, ld);
+    return k == coop_key(2, 1, 2, 1) || k == coop_key(2, 2, 1, 1) || k == coop_key(2, 2, 2, 1) ||
+           k == coop_key(3, 0, 2, 2) || k == coop_key(3, 1, 1, 1) || k == coop_key(3, 1, 2, 1) ||
+           k == coop_key(3, 2, 1, 1) || k == coop_key(3, 2, 3, 0) || k == coop_key(3, 3, 2, 0) ||
+           k == coop_key(3, 3, 3, 0);
+}
+
 template <int LA, int LB, int LC, int LD>
 struct CoopShape {
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -659,9 +674,15 @@ struct CoopShape {
     static constexpr int NW = NF <= 64 * 24 ? 1 : NF <= 128 * 24 ? 2 : 4;
     static constexpr int NT = 64 * NW;
     static constexpr int CH = (NF + NT - 1) / NT;
+    // waves per SIMD asked of the compiler: 3 for the classes whose registers
+    // otherwise land between 169 and 256 per lane (two waves) and fit 168 with
+    // at most 22 spilled; 1 (no request) for the rest
+    static constexpr int WPE = coop_three_waves(LA, LB, LC, LD) ? 3 : 1;
     // primitive quartets per phase: enough (root, direction) items to occupy the
-    // block, within 6144 doubles (48 KB) of I arrays
-    static constexpr int PB0 = NT / (NR * 3) < 6144 / SI ? NT / (NR * 3) : 6144 / SI;
+    // block, within 6144 doubles (48 KB) of I arrays, and at three waves per
+    // SIMD within the block's share of LDS (160 KB over 12 / NW blocks)
+    static constexpr int LDSI = WPE == 3 ? 1700 * NW - NDB : 6144;
+    static constexpr int PB0 = NT / (NR * 3) < LDSI / SI ? NT / (NR * 3) : LDSI / SI;
     static constexpr int PB = PB0 < 1 ? 1 : PB0;
     static constexpr int SU = PB * SI > NF ? PB * SI : NF;
 };
@@ -719,7 +740,8 @@ __device__ __forceinline__ int cart_y_rt(int l, int i)
 }
 
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT)) void k_jk_coop(const JKArgs a)
+__global__ __launch_bounds__((CoopShape<LA, LB, LC, LD>::NT))
+__attribute__((amdgpu_waves_per_eu(CoopShape<LA, LB, LC, LD>::WPE))) void k_jk_coop(const JKArgs a)
 {
     using Sh = CoopShape<LA, LB, LC, LD>;
     constexpr int NA = Sh::NA, NB = Sh::NB, NC = Sh::NC, ND = Sh::ND, NF = Sh::NF, NR = Sh::NR;
