@@ -51,6 +51,8 @@ def jk_split(la, lb, lc, ld):
     nf = na * nb * nc * nd
     if nf <= 81:
         return 1
+    if (la, lb, lc, ld) == (2, 1, 1, 1):     # 81 per lane: one wave per SIMD either way
+        return 2
     for s in range(2, na + 1):
         if na % s == 0 and nf // s <= 54:
             return s

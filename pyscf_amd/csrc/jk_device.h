@@ -962,6 +962,10 @@ __host__ __device__ constexpr int jk_split(int la, int lb, int lc, int ld)
     const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
     const int nf = na * nb * nc * nd;
     if (nf <= 81) return 1;
+    // (dp|pp): 81 per lane; at 54 its registers already exceeded 256 (one wave
+    // per SIMD), so the third slice only repeated the recurrences: C30H62
+    // cc-pVDZ class 3.40 -> 2.40 ms on MI355X (384 registers, no spills)
+    if (la == 2 && lb == 1 && lc == 1 && ld == 1) return 2;
     for (int s = 2; s <= na; ++s)   // fewest blocks of a components with at most 54 components each
         if (na % s == 0 && nf / s <= 54) return s;
     if (nf / (na * nb) <= 54) return na * nb;
