@@ -663,6 +663,29 @@ __host__ __device__ constexpr bool coop_three_waves(int la, int lb, int lc, int 
            k == coop_key(3, 3, 3, 0);
 }
 
+// Cooperative classes at two waves per SIMD by registers (169-256 per lane)
+// whose 48 KB primitive batches left LDS room for only 3-5 blocks of 64 lanes
+// per CU: batches sized to 1/8 of the CU's LDS per 64 lanes.  Measured on
+// MI355X (ms per launch): (dd|dd) C30H62 cc-pVDZ 0.257 -> 0.147; def2-TZVP
+// chain (1442 AOs) (fp|dd) 19.4 -> 10.2, (fp|fp) 20.1 -> 13.9, (ff|pp) 11.2
+// -> 7.9, (fd|dp) 38.8 -> 23.1 (with a two-wave request: its smaller batches
+// took 258 registers; the same request made the others 10-50% slower).
+__host__ __device__ constexpr bool coop_two_wave_lds(int la, int lb, int lc, int ld)
+{
+    const int k = coop_key(la, lb, lc, ld);
+    return k == coop_key(2, 2, 2, 2) || k == coop_key(3, 1, 2, 2) || k == coop_key(3, 1, 3, 1) ||
+           k == coop_key(3, 2, 2, 1) || k == coop_key(3, 2, 2, 2) || k == coop_key(3, 2, 3, 1) ||
+           k == coop_key(3, 3, 1, 1) || k == coop_key(3, 3, 2, 1);
+}
+// Cooperative classes at three waves by registers whose batches held them to
+// 2.25 (LDS): batches within 1/12 of the CU's LDS per 64 lanes ((fp|fs) 1.07x,
+// (ff|ps) 1.05x on the def2-TZVP chain).
+__host__ __device__ constexpr bool coop_three_wave_lds(int la, int lb, int lc, int ld)
+{
+    const int k = coop_key(la, lb, lc, ld);
+    return k == coop_key(3, 1, 3, 0) || k == coop_key(3, 3, 1, 0);
+}
+
 template <int LA, int LB, int LC, int LD>
 struct CoopShape {
     static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
@@ -676,12 +699,18 @@ struct CoopShape {
     static constexpr int CH = (NF + NT - 1) / NT;
     // waves per SIMD asked of the compiler: 3 for the classes whose registers
     // otherwise land between 169 and 256 per lane (two waves) and fit 168 with
-    // at most 22 spilled; 1 (no request) for the rest
-    static constexpr int WPE = coop_three_waves(LA, LB, LC, LD) ? 3 : 1;
+    // at most 22 spilled, 2 for (fd|dp) (its LDS-sized batches otherwise take
+    // 258), 1 (no request) for the rest
+    static constexpr int WPE = coop_three_waves(LA, LB, LC, LD)               ? 3
+                               : coop_key(LA, LB, LC, LD) == coop_key(3, 2, 2, 1) ? 2
+                                                                                 : 1;
     // primitive quartets per phase: enough (root, direction) items to occupy the
-    // block, within 6144 doubles (48 KB) of I arrays, and at three waves per
-    // SIMD within the block's share of LDS (160 KB over 12 / NW blocks)
-    static constexpr int LDSI = WPE == 3 ? 1700 * NW - NDB : 6144;
+    // block, within 6144 doubles (48 KB) of I arrays, and at two or three waves
+    // per SIMD within the block's share of LDS (160 KB over 8 / NW or 12 / NW
+    // blocks)
+    static constexpr int LDSI = WPE == 3 || coop_three_wave_lds(LA, LB, LC, LD) ? 1700 * NW - NDB
+                                : coop_two_wave_lds(LA, LB, LC, LD)           ? 2560 * NW - NDB
+                                                                              : 6144;
     static constexpr int PB0 = NT / (NR * 3) < LDSI / SI ? NT / (NR * 3) : LDSI / SI;
     static constexpr int PB = PB0 < 1 ? 1 : PB0;
     static constexpr int SU = PB * SI > NF ? PB * SI : NF;
