@@ -184,6 +184,37 @@ def test_cooperative_classes_dd(oracle):
     assert abs(vj - j0).max() < TOL and abs(vk - k0).max() < TOL
 
 
+def test_cooperative_density_counts(oracle):
+    """The cooperative kernel sums a bra pair's J'[a,b] over its kets in LDS for
+    one or two densities and adds per quartet for more: J and K of each density
+    agree whether it is built alone, in a pair or among three, and with the
+    oracle; the row kernels fall back to the cooperative kernel (row LDS 0) so
+    that most d-shell classes take it."""
+    print('cooperative kernel, 1-3 densities', flush=True)
+    mol = gto.M(atom='O 0 0 0; O 0 0 1.2; O 1.1 0.2 0.3', basis='def2-svp')
+    nao = mol.nao_nr()
+    rng = numpy.random.default_rng(12)
+    dms = rng.standard_normal((3, nao, nao)) / 4
+    opt = _vhf.VHFOpt(mol, direct_scf_tol=1e-13)
+    eng = _vhf.get_engine()
+    try:
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 0) == 0
+        single = [hf.get_jk(mol, dms[i], hermi=0, vhfopt=opt) for i in range(3)]
+        pair = hf.get_jk(mol, dms[:2], hermi=0, vhfopt=opt)
+        three = hf.get_jk(mol, dms, hermi=0, vhfopt=opt)
+    finally:
+        assert eng.lib.mjk_ctx_set_row_lds(eng.handle, 162816) == 0
+    for i in range(3):
+        js, ks = single[i]
+        scale = max(1.0, abs(js).max(), abs(ks).max())
+        assert abs(three[0][i] - js).max() < 1e-12 * scale and abs(three[1][i] - ks).max() < 1e-12 * scale
+        if i < 2:
+            assert abs(pair[0][i] - js).max() < 1e-12 * scale and abs(pair[1][i] - ks).max() < 1e-12 * scale
+    j0, k0 = oracle.get_jk(mol, dms[2], hermi=0, vhfopt=oracle.VHFOpt(mol, 1e-13))
+    assert abs(three[0][2] - j0).max() < TOL * max(1.0, abs(j0).max())
+    assert abs(three[1][2] - k0).max() < TOL * max(1.0, abs(k0).max())
+
+
 @pytest.mark.parametrize('row_lds', [0, 3072])
 def test_row_and_tile_kernels_agree(oracle, row_lds):
     # row kernels (K' rows in LDS) against their fallback kernels: 0 forces the fallback (tile or
